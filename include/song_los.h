@@ -1,0 +1,174 @@
+/*
+ * song_los.h — engine-level C-ABI of the MI355X LOS transfer library.
+ *
+ * transfer2.h exposes SONG's own entry points; this header exposes what sits
+ * under them, with plain C types only, for callers that want to keep inputs
+ * resident on the GPU (benchmarks, a serving loop, other language bindings):
+ *
+ *  - los_engine_*   : upload the source and projection tables of a set of
+ *                     (k1,k2) pairs once, run the LOS kernels on them as many
+ *                     times as wanted, scatter the results into ptr2->transfer.
+ *  - los_scenario_* : build seeded synthetic inputs (k grids, source tables,
+ *                     J_Llm(x) tables) in SONG's structures, for tests,
+ *                     benchmarks and users without a perturbations2 run.
+ *  - los_bessel2_build : FP64 builder of struct bessels2 (the J_Llm(x) tables
+ *                     that source/bessel2.c:75-468 produces in SONG).
+ *
+ * All functions return _SUCCESS_ (0) or _FAILURE_ (1) and write a message into
+ * the caller's ErrorMsg buffer on failure.
+ */
+#ifndef SONG_LOS_H
+#define SONG_LOS_H
+
+#include "song_los_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Kernel forms.
+ *  LOS_FORM_FP64        : one workgroup per fine k3, per-element operation
+ *                         order of source/transfer2.c:2192-2329, no FMA
+ *                         contraction: bit-identical to the FP64 CPU path.
+ *  LOS_FORM_FP64_FAST   : tiled FP64 kernel with FMA; same math, different
+ *                         rounding (1e-10 relative class).
+ *  LOS_FORM_FP32C       : source x projection products in FP32 with
+ *                         compensated (TwoSum) tau accumulation; x = k(tau0-tau),
+ *                         its table index and weight, and the spline
+ *                         coefficients stay FP64.
+ * ---------------------------------------------------------------------- */
+enum los_form { LOS_FORM_FP64 = 0, LOS_FORM_FP64_FAST = 1, LOS_FORM_FP32C = 2 };
+
+typedef struct los_engine los_engine;
+
+/* Create an engine on HIP device `device` (one stream). */
+int los_engine_create(int device, los_engine **out, char *errmsg);
+void los_engine_destroy(los_engine *eng);
+
+/* Select the kernel form used by later runs (and by transfer2_init). */
+int los_engine_set_form(los_engine *eng, int form, char *errmsg);
+
+/*
+ * Upload everything the LOS kernels need for the pairs with index_k1 in
+ * [k1_begin, k1_end) (all index_k2 <= index_k1): sources ppt2->sources,
+ * coarse and fine k3 grids, the time grid and the J tables. Requires
+ * transfer2_indices_of_transfers() to have filled ptr2. The sources are
+ * read, not freed. J tables are uploaded once per engine and kept resident.
+ */
+int los_engine_load(los_engine *eng, struct precision *ppr, struct precision2 *ppr2,
+                    struct perturbs2 *ppt2, struct bessels *pbs, struct bessels2 *pbs2,
+                    struct transfers2 *ptr2, int k1_begin, int k1_end, char *errmsg);
+
+/* Run the LOS kernels on the loaded pairs; *ms gets the GPU time (HIP events). */
+int los_engine_run(los_engine *eng, float *ms, char *errmsg);
+
+/* GPU time of the last run's main LOS kernel alone (ms) and its launch count. */
+int los_engine_last_kernel_ms(los_engine *eng, float *ms_los, float *ms_spline);
+
+/* Copy the results of the loaded pairs into ptr2->transfer[tt][k1][k2][k]
+   (levels must be allocated) and update ptr2->count_memorised_transfers. */
+int los_engine_store(los_engine *eng, struct transfers2 *ptr2, char *errmsg);
+
+/* Element count (tt x fine k3 summed over loaded pairs) and live-work count
+   (sum over elements of live (tau,L) terms, the reference's skip applied). */
+long los_engine_n_elements(los_engine *eng);
+double los_engine_live_terms(los_engine *eng);
+/* Algorithmic bytes of one run: sources read once + J window read once +
+   outputs written once. */
+double los_engine_alg_bytes(los_engine *eng);
+/* The HIP stream the engine launches on (hipStream_t as void*). */
+void *los_engine_stream(los_engine *eng);
+
+/* Process-wide default form used by transfer2_init (default LOS_FORM_FP64),
+   and the device it runs on (default 0). */
+int los_set_default_form(int form);
+int los_set_default_device(int device);
+
+/* ------------------------------------------------------------------------
+ * Synthetic inputs.
+ * ---------------------------------------------------------------------- */
+typedef struct los_config {
+  /* fields */
+  int has_T;          /* temperature */
+  int has_E;          /* E polarisation (B sources appear when m_max > 0) */
+  int m_list[_MAX_NUM_AZIMUTHAL_];
+  int m_size;
+  int l_max_los_t;    /* sources L = 0..l_max_los_t for T */
+  int l_max_los_p;    /* and for E, B */
+  /* (k1,k2) grid: k_size values, log (k_log=1) or linear spaced in [k_min,k_max] */
+  int k_size;
+  double k_min, k_max;
+  int k_log;
+  /* coarse k3 per pair: k3_size values, linear in the triangle-limited range */
+  int k3_size;
+  /* time sampling of the sources: tau_size values log-spaced in [tau_min,tau_max] */
+  int tau_size;
+  double tau_min, tau_max;
+  double tau0, tau_rec, rs_rec;
+  /* multipoles today: l_size values from 2 to l_max (log then linear) */
+  int l_size;
+  int l_max;
+  /* fine k3 step (sets q_linstep_song so that k_step_max equals it) */
+  double dk_fine;
+  /* J tables */
+  double xx_max, xx_step;
+  double J_cut, j_cut;
+  /* options of the stage */
+  int k3_interpolation;      /* enum interpolation_methods */
+  int bessels_interpolation; /* enum interpolation_methods */
+  int tau_sampling;          /* enum transfer2_tau_sampling */
+  int time_interpolation;    /* enum interpolation_methods */
+  double tau_linstep;
+  int k3_sampling;           /* enum transfer2_k3_sampling */
+  int extrapolation;         /* enum bispectra_k3_extrapolation */
+  double extra_osc_left, extra_osc_right;
+  /* seeded source model: A ~ N(0,1) per (field,L,m,k1,k2) x (k3/0.01)^-1.5 x
+     [exp(-(tau-tau_peak)^2/(2 sigma^2)) + ramp_amp * ramp(tau)] */
+  unsigned long long seed;
+  double tau_peak, tau_sigma, ramp_amp, ramp_center, ramp_width;
+  /* verbosity of the stage */
+  int verbose;
+} los_config;
+
+/* Defaults = BASELINE-style plain FP64 configuration at small size. */
+void los_config_default(los_config *cfg);
+
+typedef struct los_scenario los_scenario;
+
+int los_scenario_create(const los_config *cfg, los_scenario **out, char *errmsg);
+void los_scenario_destroy(los_scenario *s);
+void los_scenario_structs(los_scenario *s, struct precision **ppr, struct precision2 **ppr2,
+                          struct background **pba, struct thermo **pth,
+                          struct perturbs **ppt, struct perturbs2 **ppt2,
+                          struct bessels **pbs, struct bessels2 **pbs2,
+                          struct transfers **ptr, struct transfers2 **ptr2);
+
+/* Total number of values in ptr2->transfer (after transfer2_init) and a copy of
+   them in (tt, k1, k2<=k1, k) order. */
+long los_scenario_n_transfers(los_scenario *s);
+int los_scenario_gather(los_scenario *s, double *out, long n, char *errmsg);
+
+/* Set a named scalar of the structures (e.g. "store_transfers_to_disk",
+   "transfers_dir" is set with los_scenario_set_string). */
+int los_scenario_set(los_scenario *s, const char *key, double value, char *errmsg);
+int los_scenario_set_string(los_scenario *s, const char *key, const char *value,
+                            char *errmsg);
+
+/* FP64 builder of the projection-function tables of struct bessels2, with the
+   semantics of source/bessel2.c:75-468 (xx grid, index_xmin_J by bisection on
+   J_Llm_cut, rows stored from index_xmin_J, ddJ when bessels are cubic). */
+int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                      struct bessels *pbs, struct bessels2 *pbs2, char *errmsg);
+int los_bessel2_free(struct bessels *pbs, struct bessels2 *pbs2);
+
+/* Special functions used by the builder (exported for tests). */
+void los_sph_bessel_array(int lmax, double x, double *out);
+int los_wigner3j_j1(int j2, int j3, int m2, int m3, int *j1min, int *j1max, double *out,
+                    int out_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SONG_LOS_H */

@@ -1,0 +1,80 @@
+"""Build the song_los shared library (HIP for gfx950 + C host) and the CPU oracle.
+
+Usage: python -m song_amd.build [--no-oracle]
+
+Products (in-tree, git-ignored, shipped with the tree to the GPU box):
+  song_amd/libsong_los.so   the LOS stage: transfer2_* C-ABI + HIP kernels
+  oracle/liblos_oracle.so   CPU oracle (test infrastructure only)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+C_SOURCES = ["los_special.c", "los_lm.c", "los_bessel2.c", "los_scenario.c", "los_transfer2.c"]
+# (source, extra flags); the exact kernels must not contract a*b+c into FMA
+HIP_SOURCES = [
+    ("los_engine.hip", []),
+    ("los_kernels_exact.hip", ["-ffp-contract=off"]),
+]
+
+C_FLAGS = ["-O2", "-std=c99", "-D_GNU_SOURCE", "-fPIC", "-fopenmp", "-Wall",
+           "-Wno-format-truncation", "-Wno-unused-function"]
+HIP_FLAGS = ["-O3", "-fPIC", f"--offload-arch={ARCH}", "-std=c++17",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} {cmd[-1]}")
+    return r
+
+
+def build_library(out=os.path.join(HERE, "libsong_los.so")):
+    with tempfile.TemporaryDirectory(prefix="songlos_") as tmp:
+        objs = []
+        for src in C_SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+        for src, extra in HIP_SOURCES:
+            obj = os.path.join(tmp, src + ".o")
+            _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+            objs.append(obj)
+        tmp_out = out + ".tmp"
+        _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp_out, *objs, "-lgomp", "-lm"])
+        os.replace(tmp_out, out)
+    return out
+
+
+def build_oracle(out=os.path.join(ROOT, "oracle", "liblos_oracle.so")):
+    src = os.path.join(ROOT, "oracle", "los_ref.c")
+    tmp_out = out + ".tmp"
+    _run(["gcc", "-O3", "-std=c99", "-D_GNU_SOURCE", "-fopenmp", "-fPIC", "-shared",
+          "-Wno-format-truncation", "-o", tmp_out, src, "-lm"])
+    os.replace(tmp_out, out)
+    return out
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    lib = build_library()
+    print("built", lib)
+    if "--no-oracle" not in argv:
+        print("built", build_oracle())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,538 @@
+// los_engine.hip — host side of the LOS engine: device residency of the
+// source and projection tables, kernel launches, result scatter.
+//
+// Data layout in HBM (one engine, one HIP device, one stream):
+//   src, dd   per pair [tp][tau][kc] FP64 (the layout of ppt2->sources, so a
+//             pair uploads as tp2_size contiguous blocks; dd is its spline)
+//   kc, kf    coarse and fine k3 grids, concatenated per pair
+//   J         [jt][L][x][lm] FP64: the (l,m) offsets of a J row at one x are
+//             contiguous, so the lanes of a wave (one lm each) load 512 B runs
+//   xmin      [jt][L][lm] first live x index of each row
+//   out       [tt][F] FP64, scattered to ptr2->transfer[tt][k1][k2][k]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "los_device.h"
+#include "los_internal.h"
+
+extern "C" hipError_t los_launch_spline_exact(const los_pair *pairs, int npairs, const double *kc,
+                                              const double *src, double *dd, int ntau, int tp2,
+                                              hipStream_t s);
+extern "C" hipError_t los_launch_integrate_exact(const los_params *P, int nthreads, hipStream_t s);
+
+struct los_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int form = LOS_FORM_FP64;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float ms_spline = 0, ms_los = 0;
+
+  // projection tables (resident across loads)
+  const void *J_owner = nullptr;
+  double *dJ = nullptr;
+  int *dxmin = nullptr, *dminx = nullptr;
+  double *dxx = nullptr;
+  int *dl_of = nullptr, *dm_of = nullptr, *dmi_of = nullptr;
+  int X = 0, L_size = 0, J_size = 0, n_lm = 0;
+  double J_window_bytes = 0;
+
+  // time grid
+  double *dt0mt = nullptr, *ddtau = nullptr;
+  int ntau = 0;
+
+  // loaded pairs
+  std::vector<los_pair> pairs;
+  los_pair *dpairs = nullptr;
+  double *dkc = nullptr, *dkf = nullptr;
+  int *dfpair = nullptr, *dfsrc = nullptr, *dfbr = nullptr;
+  double *dsrc = nullptr, *ddd = nullptr, *dout = nullptr;
+  long src_total = 0;
+  int F = 0;
+  int tt2_size = 0;
+  los_params P;
+  bool loaded = false;
+
+  double *staging = nullptr;  // pinned
+  size_t staging_bytes = 0;
+};
+
+#define HIPCHK(expr, err)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      snprintf((err), _ERRORMSGSIZE_, "%s: %s failed: %s", __func__, #expr,                \
+               hipGetErrorString(e_));                                                     \
+      return _FAILURE_;                                                                    \
+    }                                                                                      \
+  } while (0)
+
+template <class T>
+static void dfree(T *&p)
+{
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+static void free_batch(los_engine *e)
+{
+  dfree(e->dpairs);
+  dfree(e->dkc);
+  dfree(e->dkf);
+  dfree(e->dfpair);
+  dfree(e->dfsrc);
+  dfree(e->dfbr);
+  dfree(e->dsrc);
+  dfree(e->ddd);
+  dfree(e->dout);
+  dfree(e->dt0mt);
+  dfree(e->ddtau);
+  e->loaded = false;
+}
+
+static void free_tables(los_engine *e)
+{
+  dfree(e->dJ);
+  dfree(e->dxmin);
+  dfree(e->dminx);
+  dfree(e->dxx);
+  dfree(e->dl_of);
+  dfree(e->dm_of);
+  dfree(e->dmi_of);
+  e->J_owner = nullptr;
+}
+
+extern "C" int los_engine_create(int device, los_engine **out, char *err)
+{
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n), err);
+  if (device < 0 || device >= n) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_create: no HIP device %d (%d visible)", device, n);
+    return _FAILURE_;
+  }
+  HIPCHK(hipSetDevice(device), err);
+  los_engine *e = new los_engine();
+  e->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), err);
+  for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&e->ev[i]), err);
+  e->staging_bytes = 64ul << 20;
+  HIPCHK(hipHostMalloc((void **)&e->staging, e->staging_bytes, hipHostMallocDefault), err);
+  *out = e;
+  return _SUCCESS_;
+}
+
+extern "C" void los_engine_destroy(los_engine *e)
+{
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  free_batch(e);
+  free_tables(e);
+  if (e->staging) (void)hipHostFree(e->staging);
+  for (int i = 0; i < 4; ++i)
+    if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+extern "C" int los_engine_set_form(los_engine *e, int form, char *err)
+{
+  if (form != LOS_FORM_FP64) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_set_form: form %d is not available in this build",
+             form);
+    return _FAILURE_;
+  }
+  e->form = form;
+  return _SUCCESS_;
+}
+
+extern "C" void *los_engine_stream(los_engine *e) { return (void *)e->stream; }
+
+extern "C" double los_engine_batch_budget(los_engine *e)
+{
+  size_t fr = 0, tot = 0;
+  (void)hipSetDevice(e->device);
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 8e9;
+  return 0.6 * (double)fr;
+}
+
+extern "C" double los_engine_batch_bytes(struct precision2 *ppr2, struct perturbs2 *ppt2,
+                                         struct transfers2 *ptr2, int k1_begin, int k1_end)
+{
+  (void)ppr2;
+  double b = 0;
+  for (int i1 = k1_begin; i1 < k1_end; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2) {
+      b += 2.0 * 8.0 * ppt2->tp2_size * (double)ppt2->tau_size * ppt2->k3_size[i1][i2];
+      b += 8.0 * (ptr2->tt2_size + 4.0) * ptr2->k_size_k1k2[i1][i2];
+    }
+  return b;
+}
+
+// Copy host memory to the device through the pinned staging buffer.
+static int upload(los_engine *e, void *dst, const void *src, size_t bytes, char *err)
+{
+  const char *s = (const char *)src;
+  char *d = (char *)dst;
+  while (bytes > 0) {
+    size_t n = std::min(bytes, e->staging_bytes);
+    memcpy(e->staging, s, n);
+    HIPCHK(hipMemcpyAsync(d, e->staging, n, hipMemcpyHostToDevice, e->stream), err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+    s += n;
+    d += n;
+    bytes -= n;
+  }
+  return _SUCCESS_;
+}
+
+template <class T>
+static int upload_vec(los_engine *e, T *&dptr, const std::vector<T> &v, char *err)
+{
+  dfree(dptr);
+  size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
+  HIPCHK(hipMalloc((void **)&dptr, bytes), err);
+  if (!v.empty()) return upload(e, dptr, v.data(), v.size() * sizeof(T), err);
+  return _SUCCESS_;
+}
+
+// Build the device projection table [jt][L][x][lm] for x < X.
+static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                        struct bessels2 *pbs2, struct transfers2 *ptr2, double kmax_all,
+                        char *err)
+{
+  const int J_size = pbs2->J_size, L_size = pbs2->L_size, n_lm = ptr2->n_transfers;
+  const double t0mt0 = ptr2->tau0 - ppt2->tau_sampling[0];
+  const double xmax = kmax_all * t0mt0;
+  const int ixmax = (int)(xmax / pbs2->xx_step);
+  if (ixmax + 1 > pbs2->xx_size - 1) {
+    snprintf(err, _ERRORMSGSIZE_,
+             "los_engine_load: not enough J's: k*(tau0-tau_min)=%g exceeds the x grid (xx_max=%g)",
+             xmax, pbs2->xx_max);
+    return _FAILURE_;
+  }
+  const int X = std::min(ixmax + 2, pbs2->xx_size);
+  const size_t nJ = (size_t)J_size * L_size * X * n_lm;
+  std::vector<double> hJ(nJ, 0.0);
+  std::vector<int> hxmin((size_t)J_size * L_size * n_lm, INT_MAX);
+  double window = 0;
+  for (int jt = 0; jt < J_size; ++jt)
+    for (int L = 0; L < L_size; ++L)
+      for (int il = 0; il < ptr2->l_size; ++il) {
+        const int l = ptr2->l[il];
+        const int mcount = std::min(ppr2->index_m_max[L], ppr2->index_m_max[l]) + 1;
+        for (int im = 0; im < mcount; ++im) {
+          const int lm = ptr2->lm_array[il][im];
+          const int xm = pbs2->index_xmin_J[jt][L][il][im];
+          const int xs = pbs2->x_size_J[jt][L][il][im];
+          hxmin[((size_t)jt * L_size + L) * n_lm + lm] = xm;
+          if (xm >= X) continue;
+          const double *row = pbs2->J_Llm_x[jt][L][il][im];
+          const int xend = std::min(X, xm + xs);
+          for (int x = xm; x < xend; ++x)
+            hJ[(((size_t)jt * L_size + L) * X + x) * n_lm + lm] = row[x - xm];
+          window += 8.0 * (xend - xm);
+        }
+      }
+  // smallest first-live index over the rows each (l,m) element uses
+  std::vector<int> hminx(n_lm, INT_MAX), hl(n_lm), hm(n_lm), hmi(n_lm);
+  for (int il = 0; il < ptr2->l_size; ++il)
+    for (int im = 0; im <= ppr2->index_m_max[ptr2->l[il]]; ++im) {
+      const int lm = ptr2->lm_array[il][im];
+      hl[lm] = ptr2->l[il];
+      hm[lm] = ptr2->m[im];
+      hmi[lm] = im;
+      int mn = INT_MAX;
+      auto scan = [&](int jt, int Lmax, bool need_m_nonzero) {
+        if (jt < 0) return;
+        if (need_m_nonzero && ptr2->m[im] == 0) return;
+        for (int L = 0; L <= Lmax && L < L_size; ++L) {
+          if (ptr2->m[im] > L) continue;
+          mn = std::min(mn, hxmin[((size_t)jt * L_size + L) * n_lm + lm]);
+        }
+      };
+      if (ppt2->has_source_T) scan(pbs2->index_J_TT, ppr2->l_max_los_t, false);
+      if (ppt2->has_source_E || ppt2->has_source_B) {
+        scan(pbs2->index_J_EE, ppr2->l_max_los_p, false);
+        if (pbs2->has_J_EB) scan(pbs2->index_J_EB, ppr2->l_max_los_p, true);
+      }
+      hminx[lm] = mn;
+    }
+  free_tables(e);
+  HIPCHK(hipMalloc((void **)&e->dJ, nJ * sizeof(double)), err);
+  if (upload(e, e->dJ, hJ.data(), nJ * sizeof(double), err)) return _FAILURE_;
+  if (upload_vec(e, e->dxmin, hxmin, err)) return _FAILURE_;
+  if (upload_vec(e, e->dminx, hminx, err)) return _FAILURE_;
+  if (upload_vec(e, e->dl_of, hl, err)) return _FAILURE_;
+  if (upload_vec(e, e->dm_of, hm, err)) return _FAILURE_;
+  if (upload_vec(e, e->dmi_of, hmi, err)) return _FAILURE_;
+  std::vector<double> hxx(pbs2->xx, pbs2->xx + X);
+  if (upload_vec(e, e->dxx, hxx, err)) return _FAILURE_;
+  e->X = X;
+  e->L_size = L_size;
+  e->J_size = J_size;
+  e->n_lm = n_lm;
+  e->J_owner = pbs2;
+  e->J_window_bytes = window;
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct precision2 *ppr2,
+                               struct perturbs2 *ppt2, struct bessels *pbs,
+                               struct bessels2 *pbs2, struct transfers2 *ptr2, int k1_begin,
+                               int k1_end, char *err)
+{
+  HIPCHK(hipSetDevice(e->device), err);
+  if (ptr2->tau_sampling != sources_tau_sampling) {
+    snprintf(err, _ERRORMSGSIZE_,
+             "los_engine_load: only transfer2_tau_sampling=sources is supported by this build");
+    return _FAILURE_;
+  }
+  if (ppr->bessels_interpolation != linear_interpolation) {
+    snprintf(err, _ERRORMSGSIZE_,
+             "los_engine_load: only linear bessels_interpolation is supported by this build");
+    return _FAILURE_;
+  }
+  if (ppt2->tp2_size <= 0 || ppt2->tau_size < 2) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_load: empty source table");
+    return _FAILURE_;
+  }
+  const int LmaxT = ppr2->l_max_los_t, LmaxP = ppr2->l_max_los_p;
+  if (std::max(LmaxT, LmaxP) >= LOS_MAX_L || ppr2->m_size > LOS_MAX_MI) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_load: l_max_los (%d) or m_size (%d) too large",
+             std::max(LmaxT, LmaxP), ppr2->m_size);
+    return _FAILURE_;
+  }
+  if (ptr2->n_transfers > 1024) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_load: %d (l,m) pairs exceed one workgroup",
+             ptr2->n_transfers);
+    return _FAILURE_;
+  }
+  free_batch(e);
+  const int ntau = ppt2->tau_size;
+  const int tp2 = ppt2->tp2_size;
+
+  // time grid for transfer2_tau_sampling == sources (source/transfer2.c:2405-2412, :2567-2580)
+  std::vector<double> t0mt(ntau), dtau(ntau);
+  for (int i = 0; i < ntau; ++i) t0mt[i] = ptr2->tau0 - ppt2->tau_sampling[i];
+  dtau[0] = ppt2->tau_sampling[1] - ppt2->tau_sampling[0];
+  for (int i = 1; i < ntau - 1; ++i) dtau[i] = ppt2->tau_sampling[i + 1] - ppt2->tau_sampling[i - 1];
+  dtau[ntau - 1] = ppt2->tau_sampling[ntau - 1] - ppt2->tau_sampling[ntau - 2];
+
+  // pairs, fine k3 grids and their brackets in the coarse grid
+  e->pairs.clear();
+  std::vector<double> kc, kf;
+  std::vector<int> fpair, fsrc, fbr;
+  long src_off = 0;
+  std::vector<double> klist(std::max(1, ptr2->k3_size_max));
+  for (int i1 = k1_begin; i1 < k1_end; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2) {
+      if (!ppt2->has_allocated_sources[i1]) {
+        snprintf(err, _ERRORMSGSIZE_, "los_engine_load: sources of index_k1=%d are not allocated", i1);
+        return _FAILURE_;
+      }
+      los_pair p;
+      p.k1 = i1;
+      p.k2 = i2;
+      p.nc = ppt2->k3_size[i1][i2];
+      p.kc_off = (int)kc.size();
+      p.src_off = src_off;
+      p.f_off = (int)kf.size();
+      p.nf = ptr2->k_size_k1k2[i1][i2];
+      if (p.nc > LOS_NC_MAX || p.nc < 4) {
+        snprintf(err, _ERRORMSGSIZE_, "los_engine_load: coarse k3 count %d outside [4,%d]", p.nc,
+                 LOS_NC_MAX);
+        return _FAILURE_;
+      }
+      int last_used = 0;
+      if (transfer2_get_k3_list(ppr, ppr2, ppt2, pbs, pbs2, ptr2, i1, i2, klist.data(),
+                                &last_used) == _FAILURE_) {
+        memcpy(err, ptr2->error_message, _ERRORMSGSIZE_);
+        return _FAILURE_;
+      }
+      const double *kpt = ppt2->k3[i1][i2];
+      const int first = ptr2->k_physical_start_k1k2[i1][i2];
+      const int last = first + ptr2->k_physical_size_k1k2[i1][i2] - 1;
+      // bracket walk of source/transfer2.c:2681-2686
+      std::vector<int> br(p.nf, 0);
+      int ik = 0;
+      for (int it = first; it <= last; ++it) {
+        while ((ik + 1) < p.nc && kpt[ik + 1] < klist[it]) ik++;
+        if (ik > p.nc - 2) {
+          snprintf(err, _ERRORMSGSIZE_, "los_engine_load: k3=%g beyond the coarse grid", klist[it]);
+          return _FAILURE_;
+        }
+        br[it] = ik;
+      }
+      for (int q = 0; q < p.nc; ++q) kc.push_back(kpt[q]);
+      for (int it = 0; it < p.nf; ++it) {
+        const int s = it < first ? first : (it > last ? last : it);
+        kf.push_back(klist[it]);
+        fpair.push_back((int)e->pairs.size());
+        fsrc.push_back(p.f_off + s);
+        fbr.push_back(br[s]);
+      }
+      src_off += (long)tp2 * ntau * p.nc;
+      e->pairs.push_back(p);
+    }
+  e->F = (int)kf.size();
+  e->src_total = src_off;
+  e->tt2_size = ptr2->tt2_size;
+  e->ntau = ntau;
+
+  // projection tables: built once for the whole k range of the run
+  if (e->J_owner != (const void *)pbs2) {
+    double kmax_all = 0;
+    for (int i1 = 0; i1 < ppt2->k_size; ++i1)
+      for (int i2 = 0; i2 <= i1; ++i2) kmax_all = std::max(kmax_all, ptr2->k_max_k1k2[i1][i2]);
+    if (build_tables(e, ppr2, ppt2, pbs2, ptr2, kmax_all, err) == _FAILURE_) return _FAILURE_;
+  }
+
+  if (upload_vec(e, e->dt0mt, t0mt, err) || upload_vec(e, e->ddtau, dtau, err) ||
+      upload_vec(e, e->dpairs, e->pairs, err) || upload_vec(e, e->dkc, kc, err) ||
+      upload_vec(e, e->dkf, kf, err) || upload_vec(e, e->dfpair, fpair, err) ||
+      upload_vec(e, e->dfsrc, fsrc, err) || upload_vec(e, e->dfbr, fbr, err))
+    return _FAILURE_;
+
+  // sources: per pair, tp2 contiguous [tau][kc] blocks
+  HIPCHK(hipMalloc((void **)&e->dsrc, std::max<long>(1, e->src_total) * sizeof(double)), err);
+  HIPCHK(hipMalloc((void **)&e->ddd, std::max<long>(1, e->src_total) * sizeof(double)), err);
+  for (const los_pair &p : e->pairs)
+    for (int tp = 0; tp < tp2; ++tp) {
+      const size_t n = (size_t)ntau * p.nc;
+      if (upload(e, e->dsrc + p.src_off + (long)tp * n, ppt2->sources[tp][p.k1][p.k2],
+                 n * sizeof(double), err))
+        return _FAILURE_;
+    }
+  HIPCHK(hipMalloc((void **)&e->dout, std::max<size_t>(1, (size_t)e->tt2_size * e->F) * sizeof(double)),
+         err);
+
+  // kernel parameters
+  los_params &P = e->P;
+  memset(&P, 0, sizeof P);
+  P.pairs = e->dpairs;
+  P.kc = e->dkc;
+  P.kf = e->dkf;
+  P.fpair = e->dfpair;
+  P.fsrc = e->dfsrc;
+  P.fbr = e->dfbr;
+  P.src = e->dsrc;
+  P.dd = e->ddd;
+  P.F = e->F;
+  P.tp2_size = tp2;
+  P.cubic_k3 = ppr2->sources_k3_interpolation == cubic_interpolation;
+  P.t0mt = e->dt0mt;
+  P.dtau = e->ddtau;
+  P.ntau = ntau;
+  P.J = e->dJ;
+  P.xmin = e->dxmin;
+  P.min_xmin = e->dminx;
+  P.xx = e->dxx;
+  P.xx_step = pbs2->xx_step;
+  P.X = e->X;
+  P.L_size = e->L_size;
+  P.n_lm = e->n_lm;
+  P.l_of = e->dl_of;
+  P.m_of = e->dm_of;
+  P.mi_of = e->dmi_of;
+  P.has_T = ppt2->has_source_T;
+  P.has_E = ppt2->has_source_E;
+  P.has_B = ppt2->has_source_B;
+  P.LmaxT = LmaxT;
+  P.LmaxP = LmaxP;
+  P.jTT = pbs2->has_J_TT ? pbs2->index_J_TT : -1;
+  P.jEE = pbs2->has_J_EE ? pbs2->index_J_EE : -1;
+  P.jEB = pbs2->has_J_EB ? pbs2->index_J_EB : -1;
+  P.ttT = ptr2->index_tt2_T;
+  P.ttE = ptr2->index_tt2_E;
+  P.ttB = ptr2->index_tt2_B;
+  for (int L = 0; L < LOS_MAX_L; ++L)
+    for (int im = 0; im < LOS_MAX_MI; ++im) {
+      P.srcT[L][im] = P.srcE[L][im] = P.srcB[L][im] = -1;
+      if (L > ppt2->largest_l || im >= ppr2->m_size || ppr2->m[im] > L) continue;
+      if (im > ppr2->index_m_max[L]) continue;
+      const int off = ppt2->lm_array[L][im];
+      if (ppt2->has_source_T && L <= LmaxT) P.srcT[L][im] = ppt2->index_tp2_T + off;
+      if (ppt2->has_source_E && L <= LmaxP) P.srcE[L][im] = ppt2->index_tp2_E + off;
+      if (ppt2->has_source_B && L <= LmaxP) P.srcB[L][im] = ppt2->index_tp2_B + off;
+    }
+  if ((P.has_T && P.jTT < 0) || ((P.has_E || P.has_B) && P.jEE < 0) ||
+      (P.has_B && P.jEB < 0 && ppr2->m_max_song > 0)) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_load: projection functions missing for a field");
+    return _FAILURE_;
+  }
+  P.out = e->dout;
+  e->loaded = true;
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
+{
+  if (!e->loaded) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_run: nothing loaded");
+    return _FAILURE_;
+  }
+  HIPCHK(hipSetDevice(e->device), err);
+  const int tp2 = e->P.tp2_size;
+  HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
+  if (e->P.cubic_k3)
+    HIPCHK(los_launch_spline_exact(e->dpairs, (int)e->pairs.size(), e->dkc, e->dsrc, e->ddd,
+                                   e->ntau, tp2, e->stream),
+           err);
+  HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
+  const int threads = ((e->n_lm + 63) / 64) * 64;
+  HIPCHK(los_launch_integrate_exact(&e->P, threads, e->stream), err);
+  HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
+  HIPCHK(hipEventSynchronize(e->ev[2]), err);
+  HIPCHK(hipEventElapsedTime(&e->ms_spline, e->ev[0], e->ev[1]), err);
+  HIPCHK(hipEventElapsedTime(&e->ms_los, e->ev[1], e->ev[2]), err);
+  if (ms) *ms = e->ms_spline + e->ms_los;
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_last_kernel_ms(los_engine *e, float *ms_los, float *ms_spline)
+{
+  if (ms_los) *ms_los = e->ms_los;
+  if (ms_spline) *ms_spline = e->ms_spline;
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_store(los_engine *e, struct transfers2 *ptr2, char *err)
+{
+  if (!e->loaded) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_store: nothing loaded");
+    return _FAILURE_;
+  }
+  HIPCHK(hipSetDevice(e->device), err);
+  const size_t n = (size_t)e->tt2_size * e->F;
+  std::vector<double> h(n);
+  HIPCHK(hipMemcpyAsync(h.data(), e->dout, n * sizeof(double), hipMemcpyDeviceToHost, e->stream), err);
+  HIPCHK(hipStreamSynchronize(e->stream), err);
+  for (int tt = 0; tt < e->tt2_size; ++tt)
+    for (const los_pair &p : e->pairs) {
+      if (!ptr2->has_allocated_transfers[p.k1]) {
+        snprintf(err, _ERRORMSGSIZE_, "los_engine_store: k1 level %d not allocated", p.k1);
+        return _FAILURE_;
+      }
+      memcpy(ptr2->transfer[tt][p.k1][p.k2], h.data() + (size_t)tt * e->F + p.f_off,
+             p.nf * sizeof(double));
+      ptr2->count_memorised_transfers += p.nf;
+    }
+  return _SUCCESS_;
+}
+
+extern "C" long los_engine_n_elements(los_engine *e) { return (long)e->tt2_size * e->F; }
+
+extern "C" double los_engine_live_terms(los_engine *e)
+{
+  (void)e;
+  return -1.0;
+}
+
+extern "C" double los_engine_alg_bytes(los_engine *e)
+{
+  return 8.0 * e->src_total + e->J_window_bytes + 8.0 * (double)e->tt2_size * e->F;
+}

@@ -1,0 +1,275 @@
+"""ctypes bindings of the song_los C-ABI (include/transfer2.h, include/song_los.h).
+
+The product is the shared library ``song_amd/libsong_los.so`` (HIP kernels for
+gfx950 + the C host side of SONG's transfer2 stage). This module only loads it
+and declares argument types; it never computes anything itself. Loading fails
+loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field, fields
+from typing import List, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsong_los.so")
+
+SUCCESS, FAILURE = 0, 1
+LINEAR, CUBIC = 0, 1
+NO_EXTRAP, FLAT_EXTRAP = 0, 1
+BESSEL_K3, CLASS_K3 = 0, 1
+TAU_SOURCES, TAU_CUSTOM, TAU_BESSEL = 0, 1, 2
+FORM_FP64, FORM_FP64_FAST, FORM_FP32C = 0, 1, 2
+
+ERRSZ = 2048
+MAX_M = 14
+
+
+class LosError(RuntimeError):
+    pass
+
+
+class CConfig(C.Structure):
+    """Mirror of ``los_config`` in include/song_los.h (field order matters)."""
+
+    _fields_ = [
+        ("has_T", C.c_int), ("has_E", C.c_int),
+        ("m_list", C.c_int * MAX_M), ("m_size", C.c_int),
+        ("l_max_los_t", C.c_int), ("l_max_los_p", C.c_int),
+        ("k_size", C.c_int), ("k_min", C.c_double), ("k_max", C.c_double), ("k_log", C.c_int),
+        ("k3_size", C.c_int),
+        ("tau_size", C.c_int), ("tau_min", C.c_double), ("tau_max", C.c_double),
+        ("tau0", C.c_double), ("tau_rec", C.c_double), ("rs_rec", C.c_double),
+        ("l_size", C.c_int), ("l_max", C.c_int),
+        ("dk_fine", C.c_double),
+        ("xx_max", C.c_double), ("xx_step", C.c_double),
+        ("J_cut", C.c_double), ("j_cut", C.c_double),
+        ("k3_interpolation", C.c_int), ("bessels_interpolation", C.c_int),
+        ("tau_sampling", C.c_int), ("time_interpolation", C.c_int),
+        ("tau_linstep", C.c_double), ("k3_sampling", C.c_int),
+        ("extrapolation", C.c_int), ("extra_osc_left", C.c_double), ("extra_osc_right", C.c_double),
+        ("seed", C.c_ulonglong),
+        ("tau_peak", C.c_double), ("tau_sigma", C.c_double), ("ramp_amp", C.c_double),
+        ("ramp_center", C.c_double), ("ramp_width", C.c_double),
+        ("verbose", C.c_int),
+    ]
+
+
+def _declare(lib: C.CDLL) -> C.CDLL:
+    vp, i, d, ch = C.c_void_p, C.c_int, C.c_double, C.c_char_p
+    sig = {
+        "los_config_default": (None, [C.POINTER(CConfig)]),
+        "los_scenario_create": (i, [C.POINTER(CConfig), C.POINTER(vp), ch]),
+        "los_scenario_destroy": (None, [vp]),
+        "los_scenario_structs": (None, [vp] + [C.POINTER(vp)] * 10),
+        "los_scenario_n_transfers": (C.c_long, [vp]),
+        "los_scenario_gather": (i, [vp, C.POINTER(d), C.c_long, ch]),
+        "los_scenario_set": (i, [vp, ch, d, ch]),
+        "los_scenario_set_string": (i, [vp, ch, ch, ch]),
+        "los_scenario_source_ptr": (C.POINTER(d), [vp, i, i, i]),
+        "los_scenario_J_row": (C.POINTER(d), [vp, i, i, i, i, C.POINTER(i), C.POINTER(i)]),
+        "los_sph_bessel_array": (None, [i, d, C.POINTER(d)]),
+        "los_wigner3j_j1": (i, [i, i, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(d), i]),
+        "transfer2_init": (i, [vp] * 10),
+        "transfer2_free": (i, [vp] * 3),
+        "transfer2_get_k3_list": (i, [vp] * 6 + [i, i, C.POINTER(d), C.POINTER(i)]),
+        "transfer2_load_transfers_from_disk": (i, [vp, vp, i]),
+        "transfer2_free_type_level": (i, [vp, vp, i]),
+        "los_set_default_form": (i, [i]),
+        "los_set_default_device": (i, [i]),
+        "los_engine_create": (i, [i, C.POINTER(vp), ch]),
+        "los_engine_destroy": (None, [vp]),
+        "los_engine_set_form": (i, [vp, i, ch]),
+        "los_engine_load": (i, [vp] * 7 + [i, i, ch]),
+        "los_engine_run": (i, [vp, C.POINTER(C.c_float), ch]),
+        "los_engine_last_kernel_ms": (i, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "los_engine_store": (i, [vp, vp, ch]),
+        "los_engine_n_elements": (C.c_long, [vp]),
+        "los_engine_live_terms": (d, [vp]),
+        "los_engine_alg_bytes": (d, [vp]),
+        "los_engine_stream": (vp, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: Optional[C.CDLL] = None
+
+
+def load() -> C.CDLL:
+    """Load libsong_los.so; raise if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise LosError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the LOS stage has no fallback implementation)")
+        _LIB = _declare(C.CDLL(LIB_PATH))
+    return _LIB
+
+
+def _err() -> C.Array:
+    return C.create_string_buffer(ERRSZ)
+
+
+def check(rc: int, err, what: str) -> None:
+    if rc != SUCCESS:
+        msg = err.value.decode(errors="replace") if err is not None else ""
+        raise LosError(f"{what} failed: {msg}")
+
+
+@dataclass
+class Config:
+    """Synthetic-input configuration (mirrors ``los_config``)."""
+
+    has_T: int = 1
+    has_E: int = 0
+    m_list: List[int] = field(default_factory=lambda: [0])
+    l_max_los_t: int = 4
+    l_max_los_p: int = 4
+    k_size: int = 4
+    k_min: float = 1e-4
+    k_max: float = 0.05
+    k_log: int = 1
+    k3_size: int = 12
+    tau_size: int = 64
+    tau_min: float = 200.0
+    tau_max: float = 14000.0
+    tau0: float = 14200.0
+    tau_rec: float = 280.0
+    rs_rec: float = 145.0
+    l_size: int = 8
+    l_max: int = 200
+    dk_fine: float = 2e-3
+    xx_max: float = 1000.0
+    xx_step: float = 0.3
+    J_cut: float = 1e-10
+    j_cut: float = 1e-12
+    k3_interpolation: int = CUBIC
+    bessels_interpolation: int = LINEAR
+    tau_sampling: int = TAU_SOURCES
+    time_interpolation: int = LINEAR
+    tau_linstep: float = 0.1
+    k3_sampling: int = CLASS_K3
+    extrapolation: int = NO_EXTRAP
+    extra_osc_left: float = 0.0
+    extra_osc_right: float = 0.0
+    seed: int = 20260601
+    tau_peak: float = 280.0
+    tau_sigma: float = 20.0
+    ramp_amp: float = 0.1
+    ramp_center: float = 7000.0
+    ramp_width: float = 2000.0
+    verbose: int = 0
+
+    def to_c(self) -> CConfig:
+        c = CConfig()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if f.name == "m_list":
+                for q, m in enumerate(v):
+                    c.m_list[q] = m
+                c.m_size = len(v)
+            else:
+                setattr(c, f.name, v)
+        return c
+
+
+STRUCT_NAMES = ("ppr", "ppr2", "pba", "pth", "ppt", "ppt2", "pbs", "pbs2", "ptr", "ptr2")
+
+
+class Scenario:
+    """A set of SONG input structures filled by the seeded generator."""
+
+    def __init__(self, cfg: Config):
+        self.lib = load()
+        self.cfg = cfg
+        self._c = cfg.to_c()
+        self.ptr = C.c_void_p()
+        err = _err()
+        check(self.lib.los_scenario_create(C.byref(self._c), C.byref(self.ptr), err), err,
+              "los_scenario_create")
+        ptrs = [C.c_void_p() for _ in range(10)]
+        self.lib.los_scenario_structs(self.ptr, *[C.byref(p) for p in ptrs])
+        self.s = dict(zip(STRUCT_NAMES, ptrs))
+        self._inited = False
+
+    def structs(self):
+        return [self.s[n] for n in STRUCT_NAMES]
+
+    def set(self, key: str, value: float) -> None:
+        err = _err()
+        check(self.lib.los_scenario_set(self.ptr, key.encode(), float(value), err), err, key)
+
+    def set_string(self, key: str, value: str) -> None:
+        err = _err()
+        check(self.lib.los_scenario_set_string(self.ptr, key.encode(), value.encode(), err),
+              err, key)
+
+    def transfer2_init(self, fn=None) -> None:
+        """Run transfer2_init (the product) or another function with its signature."""
+        fn = fn or self.lib.transfer2_init
+        rc = fn(*self.structs())
+        self._inited = True
+        if rc != SUCCESS:
+            raise LosError("transfer2_init failed: " + self.error_message())
+
+    def error_message(self) -> str:
+        # ptr2->error_message lives inside struct transfers2; read it via offset
+        return _transfers2_error(self.s["ptr2"])
+
+    def gather(self) -> np.ndarray:
+        n = self.lib.los_scenario_n_transfers(self.ptr)
+        out = np.empty(n, dtype=np.float64)
+        err = _err()
+        check(self.lib.los_scenario_gather(self.ptr, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                           n, err), err, "gather")
+        return out
+
+    def free_transfers(self, fn=None) -> None:
+        if self._inited:
+            fn = fn or self.lib.transfer2_free
+            fn(self.s["ppr2"], self.s["ppt2"], self.s["ptr2"])
+            self._inited = False
+
+    def close(self) -> None:
+        if self.ptr:
+            self.free_transfers()
+            self.lib.los_scenario_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _transfers2_error(ptr2: C.c_void_p) -> str:
+    lib = load()
+    off = lib_offsets()["transfers2.error_message"]
+    buf = C.string_at(ptr2.value + off, ERRSZ)
+    return buf.split(b"\0", 1)[0].decode(errors="replace")
+
+
+_OFFS = None
+
+
+def lib_offsets():
+    global _OFFS
+    if _OFFS is None:
+        lib = load()
+        fn = lib.los_struct_offsets
+        fn.restype = C.c_int
+        fn.argtypes = [C.POINTER(C.c_long), C.c_int]
+        buf = (C.c_long * 16)()
+        fn(buf, 16)
+        _OFFS = {"transfers2.error_message": buf[0], "transfers2.tt2_size": buf[1],
+                 "transfers2.n_transfers": buf[2]}
+    return _OFFS
