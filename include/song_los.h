@@ -80,6 +80,15 @@ double los_engine_alg_bytes(los_engine *eng);
 /* The HIP stream the engine launches on (hipStream_t as void*). */
 void *los_engine_stream(los_engine *eng);
 
+/* The part of transfer2_init that precedes the integral: copies tau0, tau_rec
+   and rs_rec, runs transfer2_indices_of_transfers (l, m, k3 bookkeeping) and
+   allocates every k1 level of ptr2->transfer, so that engine runs can be
+   stored with los_engine_store. Free with transfer2_free. */
+int los_transfer2_prepare(struct precision *ppr, struct precision2 *ppr2, struct background *pba,
+                          struct thermo *pth, struct perturbs *ppt, struct perturbs2 *ppt2,
+                          struct bessels *pbs, struct bessels2 *pbs2, struct transfers *ptr,
+                          struct transfers2 *ptr2);
+
 /* Process-wide default form used by transfer2_init (default LOS_FORM_FP64),
    and the device it runs on (default 0). */
 int los_set_default_form(int form);

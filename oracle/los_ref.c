@@ -916,6 +916,134 @@ int los_ref_transfer2_free(struct precision2 *ppr2, struct perturbs2 *ppt2, stru
   return _SUCCESS_;
 }
 
+/*
+ * Bounded-sample timing entry for the CPU baseline: the oracle's bookkeeping
+ * (its own transfers2, filled here) and then, for each listed pair, the
+ * reference loop body (k3 list, k3 interpolation, OpenMP static loop over k3,
+ * compute for every tt). Results go to a flat buffer instead of ptr2->transfer.
+ * Returns the number of elements computed in *n_out. Does not free sources.
+ */
+long los_ref_sizeof_transfers2(void) { return (long)sizeof(struct transfers2); }
+
+int los_ref_run_pairs(struct precision *ppr, struct precision2 *ppr2, struct background *pba,
+                      struct thermo *pth, struct perturbs2 *ppt2, struct bessels *pbs,
+                      struct bessels2 *pbs2, struct transfers2 *ptr2, const int *k1s,
+                      const int *k2s, int npairs, long *n_out)
+{
+  ptr2->tau0 = pba->conformal_age;
+  ptr2->tau_rec = pth->tau_rec;
+  ptr2->rs_rec = pth->rs_rec;
+  ptr2->has_cls = _TRUE_;
+  if (ref_indices_of_transfers(ppr, ppr2, ppt2, pbs, pbs2, ptr2) == _FAILURE_) return _FAILURE_;
+  r_test(ptr2->tau_sampling != sources_tau_sampling, ptr2->error_message,
+         "sample timing supports sources tau sampling only");
+  int nth = 1;
+#ifdef _OPENMP
+#pragma omp parallel
+#pragma omp master
+  nth = omp_get_num_threads();
+#endif
+  const int T = ppt2->tau_size;
+  struct transfer2_workspace **ppw = calloc(nth, sizeof *ppw);
+  for (int th = 0; th < nth; ++th) {
+    struct transfer2_workspace *w = calloc(1, sizeof *w);
+    w->k_grid = malloc(ptr2->k3_size_max * sizeof(double));
+    w->tau_grid = malloc(T * sizeof(double));
+    w->tau0_minus_tau = malloc(T * sizeof(double));
+    w->delta_tau = malloc(T * sizeof(double));
+    w->index_tau_left = malloc(T * sizeof(int));
+    w->sources_time_spline = malloc(ppt2->tp2_size * sizeof(double *));
+    w->interpolated_sources_in_time = malloc(ppt2->tp2_size * sizeof(double *));
+    for (int tp = 0; tp < ppt2->tp2_size; ++tp) {
+      w->sources_time_spline[tp] = malloc(T * sizeof(double));
+      w->interpolated_sources_in_time[tp] = malloc(T * sizeof(double));
+    }
+    ppw[th] = w;
+  }
+  double **spl = malloc(ppt2->tp2_size * sizeof(double *));
+  double **isk = malloc(ppt2->tp2_size * sizeof(double *));
+  long count = 0;
+  for (int q = 0; q < npairs; ++q) {
+    const int i1 = k1s[q], i2 = k2s[q];
+    /* scratch output levels for this pair only */
+    double ***save = malloc(ptr2->tt2_size * sizeof(double **));
+    for (int tt = 0; tt < ptr2->tt2_size; ++tt) {
+      save[tt] = ptr2->transfer[tt][i1];
+      ptr2->transfer[tt][i1] = calloc(i1 + 1, sizeof(double *));
+      ptr2->transfer[tt][i1][i2] = malloc(ptr2->k_size_k1k2[i1][i2] * sizeof(double));
+    }
+    int last_used;
+    double *kg = malloc(ptr2->k3_size_max * sizeof(double));
+    if (los_ref_get_k3_list(ppr, ppr2, ppt2, pbs2, ptr2, i1, i2, kg, &last_used) == _FAILURE_)
+      return _FAILURE_;
+    for (int th = 0; th < nth; ++th)
+      memcpy(ppw[th]->k_grid, kg, ptr2->k_size_k1k2[i1][i2] * sizeof(double));
+    free(kg);
+    for (int tp = 0; tp < ppt2->tp2_size; ++tp) {
+      spl[tp] = malloc((size_t)ppt2->k3_size[i1][i2] * T * sizeof(double));
+      isk[tp] = malloc((size_t)ptr2->k_size_k1k2[i1][i2] * T * sizeof(double));
+      if (ref_interpolate_sources_in_k(ppr, ppr2, ppt2, ptr2, i1, i2, tp, ppw[0]->k_grid, spl[tp],
+                                       isk[tp]) == _FAILURE_)
+        return _FAILURE_;
+    }
+    int abort_flag = 0;
+#pragma omp parallel shared(ppw, abort_flag)
+    {
+      int thread = 0;
+#ifdef _OPENMP
+      thread = omp_get_thread_num();
+#endif
+      struct transfer2_workspace *pw = ppw[thread];
+#pragma omp for schedule(static)
+      for (int ik = 0; ik < ptr2->k_size_k1k2[i1][i2]; ++ik) {
+        pw->index_k = ik;
+        pw->k = pw->k_grid[ik];
+        if (ref_get_time_grid(ppr2, ppt2, pbs2, ptr2, pw) == _FAILURE_) { abort_flag = 1; continue; }
+        for (int tp = 0; tp < ppt2->tp2_size; ++tp)
+          ref_interpolate_sources_in_time(ppr2, ppt2, ptr2, isk[tp], pw->sources_time_spline[tp],
+                                          pw->interpolated_sources_in_time[tp], pw, NULL);
+        for (int tt = 0; tt < ptr2->tt2_size; tt++)
+          if (ref_compute(ppr, ppr2, ppt2, pbs2, ptr2, i1, i2, ik, ptr2->corresponding_index_l[tt],
+                          ptr2->corresponding_index_m[tt], tt, pw->interpolated_sources_in_time,
+                          pw) == _FAILURE_)
+            abort_flag = 1;
+      }
+    }
+    for (int tp = 0; tp < ppt2->tp2_size; ++tp) {
+      free(spl[tp]);
+      free(isk[tp]);
+    }
+    for (int tt = 0; tt < ptr2->tt2_size; ++tt) {
+      free(ptr2->transfer[tt][i1][i2]);
+      free(ptr2->transfer[tt][i1]);
+      ptr2->transfer[tt][i1] = save[tt];
+    }
+    free(save);
+    if (abort_flag) return _FAILURE_;
+    count += (long)ptr2->tt2_size * ptr2->k_size_k1k2[i1][i2];
+  }
+  free(spl);
+  free(isk);
+  for (int th = 0; th < nth; ++th) {
+    struct transfer2_workspace *w = ppw[th];
+    for (int tp = 0; tp < ppt2->tp2_size; ++tp) {
+      free(w->sources_time_spline[tp]);
+      free(w->interpolated_sources_in_time[tp]);
+    }
+    free(w->sources_time_spline);
+    free(w->interpolated_sources_in_time);
+    free(w->k_grid);
+    free(w->tau_grid);
+    free(w->tau0_minus_tau);
+    free(w->delta_tau);
+    free(w->index_tau_left);
+    free(w);
+  }
+  free(ppw);
+  *n_out = count;
+  return _SUCCESS_;
+}
+
 /* The oracle's (l,m) helpers, exported for their fixture tests. */
 int los_ref_size_indexl_indexm(const int *lv, int ls, const int *mv, int ms)
 {

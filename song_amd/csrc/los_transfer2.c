@@ -692,6 +692,24 @@ int transfer2_init(struct precision *ppr, struct precision2 *ppr2, struct backgr
   return _SUCCESS_;
 }
 
+int los_transfer2_prepare(struct precision *ppr, struct precision2 *ppr2, struct background *pba,
+                          struct thermo *pth, struct perturbs *ppt, struct perturbs2 *ppt2,
+                          struct bessels *pbs, struct bessels2 *pbs2, struct transfers *ptr,
+                          struct transfers2 *ptr2)
+{
+  (void)ppt;
+  ptr2->has_cls = _TRUE_;
+  ptr2->tau0 = pba->conformal_age;
+  ptr2->tau_rec = pth->tau_rec;
+  ptr2->rs_rec = pth->rs_rec;
+  los_call(transfer2_indices_of_transfers(ppr, ppr2, ppt2, pbs, pbs2, ptr, ptr2),
+           ptr2->error_message, ptr2->error_message);
+  for (int i1 = 0; i1 < ppt2->k_size; ++i1)
+    los_call(transfer2_allocate_k1_level(ppt2, ptr2, i1), ptr2->error_message,
+             ptr2->error_message);
+  return _SUCCESS_;
+}
+
 int transfer2_free(struct precision2 *ppr2, struct perturbs2 *ppt2, struct transfers2 *ptr2)
 {
   if (ptr2->has_cls != _TRUE_) return _SUCCESS_;

@@ -74,6 +74,7 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_sph_bessel_array": (None, [i, d, C.POINTER(d)]),
         "los_wigner3j_j1": (i, [i, i, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(d), i]),
         "transfer2_init": (i, [vp] * 10),
+        "los_transfer2_prepare": (i, [vp] * 10),
         "transfer2_free": (i, [vp] * 3),
         "transfer2_get_k3_list": (i, [vp] * 6 + [i, i, C.POINTER(d), C.POINTER(i)]),
         "transfer2_load_transfers_from_disk": (i, [vp, vp, i]),
