@@ -1,6 +1,6 @@
 """Benchmark of the LOS transfer stage on one MI355X.
 
-Workload (the plain FP64 GPU drop-in): T+E(+B) with m = 0,1,2 and
+Workload c2 (the plain FP64 GPU drop-in): T+E(+B) with m = 0,1,2 and
 sources L = 0..6; 32x32 log (k1,k2) grid in [1e-5, 0.1]/Mpc (528 pairs k2 <= k1), 60
 linear coarse k3 per pair, fine k3 step 1.5e-4/Mpc; 600 tau log-spaced in [200, 14000] Mpc,
 tau0 = 14200 Mpc; 96 l in [2, 2000]; J tables on x in [0, 15000] step 0.3. Seeded synthetic
@@ -20,6 +20,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -94,19 +96,39 @@ def cpu_baseline(cfg: L.Config, target_s: float):
                       f"({total_el} elements, {total_t:.1f} s, OpenMP static over k3 per pair)"}
 
 
+def row_normalised_errors(got, ref, sizes, tt2_size):
+    """|got - ref| / max|ref| over the k3 row of the same (tt, pair) (the transfer
+    functions oscillate in k3, so single zero crossings are not used as scales)."""
+    err = np.zeros_like(ref)
+    o = 0
+    for _tt in range(tt2_size):
+        for n in sizes:
+            r = ref[o:o + n]
+            scale = np.max(np.abs(r))
+            if scale > 0:
+                err[o:o + n] = np.abs(got[o:o + n] - r) / scale
+            o += n
+    return err
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--form", default="fp64", choices=sorted(FORMS))
+    ap.add_argument("--form", default="fp64", choices=sorted(FORMS),
+                    help="form reported as the headline value")
+    ap.add_argument("--also", default="fp32c,fp64fast",
+                    help="other forms timed on the same resident inputs and checked "
+                         "against the fp64 form ('' for none)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--errors-out", default="", help="write per-(l,m) errors as JSON here")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs the LOS stage on one GPU; --gpus {args.gpus} is not "
-              "supported (pairs shard across GPUs with no communication; see DESIGN.md)",
+              "supported (the (k1,k2) pairs would shard across GPUs with no communication)",
               file=sys.stderr)
         return 2
 
@@ -122,43 +144,81 @@ def main(argv=None):
     scen._inited = True
     eng = C.c_void_p()
     L.check(lib.los_engine_create(0, C.byref(eng), err), err, "engine_create")
-    L.check(lib.los_engine_set_form(eng, FORMS[args.form], err), err, "set_form")
     t0 = time.time()
     L.check(lib.los_engine_load(eng, s["ppr"], s["ppr2"], s["ppt2"], s["pbs"], s["pbs2"],
                                 s["ptr2"], 0, cfg.k_size, err), err, "engine_load")
     print(f"# tables uploaded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     n_el = lib.los_engine_n_elements(eng)
-    ms = C.c_float()
-    for _ in range(args.warmup):
-        L.check(lib.los_engine_run(eng, C.byref(ms), err), err, "engine_run")
-    los_ms, spl_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        L.check(lib.los_engine_run(eng, C.byref(ms), err), err, "engine_run")
-        a, b = C.c_float(), C.c_float()
-        lib.los_engine_last_kernel_ms(eng, C.byref(a), C.byref(b))
-        los_ms.append(a.value)
-        spl_ms.append(b.value)
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
     alg_bytes = lib.los_engine_alg_bytes(eng)
-    los_avg = sum(los_ms) / len(los_ms)
-    achieved = alg_bytes / (los_avg * 1e-3) / 1e9
-    lib.los_engine_destroy(eng)
-    scen.close()
 
+    def run_form(form):
+        L.check(lib.los_engine_set_form(eng, FORMS[form], err), err, "set_form")
+        ms = C.c_float()
+        for _ in range(args.warmup):
+            L.check(lib.los_engine_run(eng, C.byref(ms), err), err, "engine_run")
+        los_ms, spl_ms = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            L.check(lib.los_engine_run(eng, C.byref(ms), err), err, "engine_run")
+            a, b = C.c_float(), C.c_float()
+            lib.los_engine_last_kernel_ms(eng, C.byref(a), C.byref(b))
+            los_ms.append(a.value)
+            spl_ms.append(b.value)
+        wall = time.perf_counter() - t0
+        L.check(lib.los_engine_store(eng, s["ptr2"], err), err, "engine_store")
+        out = scen.gather()
+        ms_step = 1e3 * wall / args.steps
+        los_avg = sum(los_ms) / len(los_ms)
+        print(f"# {form}: {ms_step:.2f} ms/step, LOS kernel {los_avg:.2f} ms", file=sys.stderr,
+              flush=True)
+        return {"ms_per_step": ms_step, "value": n_el / (ms_step * 1e-3), "kernel_ms": los_avg,
+                "prep_ms": sum(spl_ms) / len(spl_ms)}, out
+
+    forms = [args.form] + [f for f in args.also.split(",") if f and f != args.form]
+    if "fp64" not in forms:
+        forms.append("fp64")
+    results, outs = {}, {}
+    for f in forms:
+        results[f], outs[f] = run_form(f)
+    lib.los_engine_destroy(eng)
+
+    sizes = [int(r[0]) for r in scen.k3_sizes()]
+    tt2 = len(outs["fp64"]) // sum(sizes)
+    lm_count = tt2 // (2 * cfg.has_E * (1 if max(cfg.m_list) > 0 else 0) + cfg.has_E + cfg.has_T)
+    per_lm = {}
+    for f in forms:
+        if f == "fp64":
+            continue
+        e = row_normalised_errors(outs[f], outs["fp64"], sizes, tt2)
+        results[f]["max_rel_err_vs_fp64"] = float(e.max())
+        results[f]["rms_rel_err_vs_fp64"] = float(np.sqrt(np.mean(e ** 2)))
+        per_tt = e.reshape(tt2, -1)
+        worst = int(np.argmax(per_tt.max(axis=1)))
+        results[f]["worst_tt"] = worst
+        per_lm[f] = {"max": per_tt.max(axis=1).tolist(),
+                     "rms": np.sqrt(np.mean(per_tt ** 2, axis=1)).tolist()}
+    scen.close()
+    if args.errors_out:
+        with open(args.errors_out, "w") as fh:
+            json.dump({"per_tt": per_lm, "tt2_size": tt2, "n_lm": lm_count}, fh)
+
+    head = results[args.form]
+    achieved = alg_bytes / (head["kernel_ms"] * 1e-3) / 1e9
+    kernel_name = {"fp64": "los_integrate_exact", "fp64fast": "los_integrate_stream<StreamFP64>",
+                   "fp32c": "los_integrate_stream<StreamFP32C>"}[args.form]
     line = {
-        "metric": METRIC, "value": n_el / (ms_per_step * 1e-3), "unit": "elements/s",
+        "metric": METRIC, "value": head["value"], "unit": "elements/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if args.form != "fp32c" else "f32-compensated",
         "data": "synthetic (seeded source model, FP64 J tables built on the host)",
         "config": {"workload": f"{args.workload}: {DESCR[args.workload]}", "form": args.form,
                    "elements": n_el, "pairs": cfg.k_size * (cfg.k_size + 1) // 2},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "los_integrate_exact", "kernel_ms": los_avg,
-                     "spline_ms": sum(spl_ms) / len(spl_ms), "alg_bytes": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel_name,
+                     "kernel_ms": head["kernel_ms"], "prep_ms": head["prep_ms"],
+                     "alg_bytes": alg_bytes},
+        "forms": results,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)

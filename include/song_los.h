@@ -140,7 +140,7 @@ typedef struct los_config {
   int verbose;
 } los_config;
 
-/* Defaults = BASELINE-style plain FP64 configuration at small size. */
+/* Defaults: a small plain FP64 temperature-only configuration (m = 0, L = 0..4, 4x4 pairs). */
 void los_config_default(los_config *cfg);
 
 typedef struct los_scenario los_scenario;
@@ -163,6 +163,23 @@ int los_scenario_gather(los_scenario *s, double *out, long n, char *errmsg);
 int los_scenario_set(los_scenario *s, const char *key, double value, char *errmsg);
 int los_scenario_set_string(los_scenario *s, const char *key, const char *value,
                             char *errmsg);
+
+/* Input overrides, for tests that need hand-chosen grids or analytic tables:
+   the (k1,k2) values (same count), one pair's coarse k3 grid (its sources are
+   reset to zero with the new size), one J row stored from index xmin to the end
+   of the x grid. Plus the inverse of los_scenario_gather and a readout of the
+   k3 bookkeeping (k_size, physical start, physical size per pair). */
+int los_scenario_grids(los_scenario *s, int *l_out, double *k_out, double *tau_out);
+int los_scenario_override_k(los_scenario *s, const double *k, int n, char *errmsg);
+int los_scenario_override_k3(los_scenario *s, int index_k1, int index_k2, const double *k3,
+                             int n, char *errmsg);
+int los_scenario_override_J(los_scenario *s, int index_J, int index_L, int index_l,
+                            int index_m, int xmin, const double *values, int n, char *errmsg);
+int los_scenario_scatter(los_scenario *s, const double *in, long n, char *errmsg);
+int los_scenario_k3_sizes(los_scenario *s, int *out, long n, char *errmsg);
+double *los_scenario_source_ptr(los_scenario *s, int index_tp, int index_k1, int index_k2);
+double *los_scenario_J_row(los_scenario *s, int index_J, int index_L, int index_l, int index_m,
+                           int *xmin, int *xsize);
 
 /* FP64 builder of the projection-function tables of struct bessels2, with the
    semantics of source/bessel2.c:75-468 (xx grid, index_xmin_J by bisection on

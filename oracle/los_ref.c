@@ -925,6 +925,18 @@ int los_ref_transfer2_free(struct precision2 *ppr2, struct perturbs2 *ppt2, stru
  */
 long los_ref_sizeof_transfers2(void) { return (long)sizeof(struct transfers2); }
 
+/* The oracle's bookkeeping alone (l, m lists, k3 sizes), for fixture tests. */
+int los_ref_indices(struct precision *ppr, struct precision2 *ppr2, struct background *pba,
+                    struct thermo *pth, struct perturbs2 *ppt2, struct bessels *pbs,
+                    struct bessels2 *pbs2, struct transfers2 *ptr2)
+{
+  ptr2->has_cls = _TRUE_;
+  ptr2->tau0 = pba->conformal_age;
+  ptr2->tau_rec = pth->tau_rec;
+  ptr2->rs_rec = pth->rs_rec;
+  return ref_indices_of_transfers(ppr, ppr2, ppt2, pbs, pbs2, ptr2);
+}
+
 int los_ref_run_pairs(struct precision *ppr, struct precision2 *ppr2, struct background *pba,
                       struct thermo *pth, struct perturbs2 *ppt2, struct bessels *pbs,
                       struct bessels2 *pbs2, struct transfers2 *ptr2, const int *k1s,

@@ -2,7 +2,7 @@
 
 Usage: python -m song_amd.build [--no-oracle]
 
-Products (in-tree, git-ignored, shipped with the tree to the GPU box):
+Products (in-tree and git-ignored, so they travel with a copy of the tree):
   song_amd/libsong_los.so   the LOS stage: transfer2_* C-ABI + HIP kernels
   oracle/liblos_oracle.so   CPU oracle (test infrastructure only)
 """
@@ -25,6 +25,7 @@ C_SOURCES = ["los_special.c", "los_lm.c", "los_bessel2.c", "los_scenario.c", "lo
 HIP_SOURCES = [
     ("los_engine.hip", []),
     ("los_kernels_exact.hip", ["-ffp-contract=off"]),
+    ("los_kernels_fast.hip", []),
 ]
 
 C_FLAGS = ["-O2", "-std=c99", "-D_GNU_SOURCE", "-fPIC", "-fopenmp", "-Wall",

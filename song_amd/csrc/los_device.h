@@ -5,9 +5,10 @@
 #ifndef SONG_LOS_DEVICE_H
 #define SONG_LOS_DEVICE_H
 
-#define LOS_MAX_L 12      /* sources L = 0..LOS_MAX_L-1 */
+#define LOS_MAX_L 8       /* sources L = 0..LOS_MAX_L-1 */
 #define LOS_MAX_MI 6      /* entries of the m-list */
 #define LOS_NC_MAX 256    /* coarse k3 values per pair handled by the spline kernel */
+#define LOS_EXACT_BLOCK_MAX 512 /* (l,m) lanes per workgroup of the exact LOS kernel */
 
 /* One (k1,k2) pair as laid out on the device. */
 typedef struct {
@@ -31,6 +32,12 @@ typedef struct {
   const double *src;    /* sources, per pair [tp][tau][kc] */
   const double *dd;     /* spline second derivatives, same layout */
   int F;                /* fine k3 total */
+  /* Work order: the fine k3 sorted by value (all pairs together), dealt out
+     so that each XCD of the chip receives a contiguous run of that order:
+     blocks that run together on one XCD then read neighbouring J rows and
+     share its L2. forder[s] is the fine index at sorted position s. */
+  const int *forder;
+  int f_per_xcd;        /* sorted positions per XCD (ceil(F/8)) */
   int tp2_size;
   int cubic_k3;         /* 1: cubic spline in k3, 0: linear */
 
@@ -41,6 +48,7 @@ typedef struct {
 
   /* projection tables: J[((jt*L_size + L)*X + x)*n_lm + lm] */
   const double *J;
+  const float *J32;     /* FP32 copy of J (FP32-compensated form only) */
   const int *xmin;      /* [(jt*L_size + L)*n_lm + lm], huge when the row is absent */
   const int *min_xmin;  /* [lm]: smallest xmin over the rows an element uses */
   const double *xx;     /* x grid, X entries */

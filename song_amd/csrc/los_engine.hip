@@ -19,10 +19,16 @@
 #include "los_device.h"
 #include "los_internal.h"
 
-extern "C" hipError_t los_launch_spline_exact(const los_pair *pairs, int npairs, const double *kc,
-                                              const double *src, double *dd, int ntau, int tp2,
-                                              hipStream_t s);
+extern "C" hipError_t los_launch_transpose(const los_pair *pairs, int npairs, int nc_max,
+                                           const double *raw, double *out, int ntau, int tp2,
+                                           hipStream_t s);
+extern "C" hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double *kc,
+                                            const double *spl, const double *src, double *dd,
+                                            int ntau, int tp2, hipStream_t s);
 extern "C" hipError_t los_launch_integrate_exact(const los_params *P, int nthreads, hipStream_t s);
+extern "C" hipError_t los_launch_stream_fp64(const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_stream_fp32c(const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_round_j32(const double *src, float *dst, long n, hipStream_t s);
 
 struct los_engine {
   int device = 0;
@@ -34,6 +40,8 @@ struct los_engine {
   // projection tables (resident across loads)
   const void *J_owner = nullptr;
   double *dJ = nullptr;
+  float *dJ32 = nullptr;
+  size_t nJ = 0;
   int *dxmin = nullptr, *dminx = nullptr;
   double *dxx = nullptr;
   int *dl_of = nullptr, *dm_of = nullptr, *dmi_of = nullptr;
@@ -47,10 +55,11 @@ struct los_engine {
   // loaded pairs
   std::vector<los_pair> pairs;
   los_pair *dpairs = nullptr;
-  double *dkc = nullptr, *dkf = nullptr;
-  int *dfpair = nullptr, *dfsrc = nullptr, *dfbr = nullptr;
-  double *dsrc = nullptr, *ddd = nullptr, *dout = nullptr;
+  double *dkc = nullptr, *dkf = nullptr, *dspl = nullptr;
+  int *dfpair = nullptr, *dfsrc = nullptr, *dfbr = nullptr, *dforder = nullptr;
+  double *draw = nullptr, *dsrc = nullptr, *ddd = nullptr, *dout = nullptr;
   long src_total = 0;
+  int nc_max = 0;
   int F = 0;
   int tt2_size = 0;
   los_params P;
@@ -85,6 +94,9 @@ static void free_batch(los_engine *e)
   dfree(e->dfpair);
   dfree(e->dfsrc);
   dfree(e->dfbr);
+  dfree(e->dforder);
+  dfree(e->dspl);
+  dfree(e->draw);
   dfree(e->dsrc);
   dfree(e->ddd);
   dfree(e->dout);
@@ -96,6 +108,7 @@ static void free_batch(los_engine *e)
 static void free_tables(los_engine *e)
 {
   dfree(e->dJ);
+  dfree(e->dJ32);
   dfree(e->dxmin);
   dfree(e->dminx);
   dfree(e->dxx);
@@ -140,9 +153,8 @@ extern "C" void los_engine_destroy(los_engine *e)
 
 extern "C" int los_engine_set_form(los_engine *e, int form, char *err)
 {
-  if (form != LOS_FORM_FP64) {
-    snprintf(err, _ERRORMSGSIZE_, "los_engine_set_form: form %d is not available in this build",
-             form);
+  if (form != LOS_FORM_FP64 && form != LOS_FORM_FP64_FAST && form != LOS_FORM_FP32C) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_set_form: unknown form %d", form);
     return _FAILURE_;
   }
   e->form = form;
@@ -246,18 +258,20 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
       hm[lm] = ptr2->m[im];
       hmi[lm] = im;
       int mn = INT_MAX;
-      auto scan = [&](int jt, int Lmax, bool need_m_nonzero) {
+      auto scan = [&](int jt, int Lmax) {
         if (jt < 0) return;
-        if (need_m_nonzero && ptr2->m[im] == 0) return;
         for (int L = 0; L <= Lmax && L < L_size; ++L) {
           if (ptr2->m[im] > L) continue;
           mn = std::min(mn, hxmin[((size_t)jt * L_size + L) * n_lm + lm]);
         }
       };
-      if (ppt2->has_source_T) scan(pbs2->index_J_TT, ppr2->l_max_los_t, false);
+      // rows an element reads: TT for T; EE for E and B; EB for B (any m) and
+      // for E when m != 0
+      if (ppt2->has_source_T) scan(pbs2->index_J_TT, ppr2->l_max_los_t);
       if (ppt2->has_source_E || ppt2->has_source_B) {
-        scan(pbs2->index_J_EE, ppr2->l_max_los_p, false);
-        if (pbs2->has_J_EB) scan(pbs2->index_J_EB, ppr2->l_max_los_p, true);
+        scan(pbs2->index_J_EE, ppr2->l_max_los_p);
+        if (pbs2->has_J_EB && (ppt2->has_source_B || ptr2->m[im] != 0))
+          scan(pbs2->index_J_EB, ppr2->l_max_los_p);
       }
       hminx[lm] = mn;
     }
@@ -275,6 +289,7 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
   e->L_size = L_size;
   e->J_size = J_size;
   e->n_lm = n_lm;
+  e->nJ = nJ;
   e->J_owner = pbs2;
   e->J_window_bytes = window;
   return _SUCCESS_;
@@ -306,11 +321,6 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
              std::max(LmaxT, LmaxP), ppr2->m_size);
     return _FAILURE_;
   }
-  if (ptr2->n_transfers > 1024) {
-    snprintf(err, _ERRORMSGSIZE_, "los_engine_load: %d (l,m) pairs exceed one workgroup",
-             ptr2->n_transfers);
-    return _FAILURE_;
-  }
   free_batch(e);
   const int ntau = ppt2->tau_size;
   const int tp2 = ppt2->tp2_size;
@@ -324,7 +334,8 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
 
   // pairs, fine k3 grids and their brackets in the coarse grid
   e->pairs.clear();
-  std::vector<double> kc, kf;
+  e->nc_max = 0;
+  std::vector<double> kc, kf, spl;
   std::vector<int> fpair, fsrc, fbr;
   long src_off = 0;
   std::vector<double> klist(std::max(1, ptr2->k3_size_max));
@@ -368,6 +379,21 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
         br[it] = ik;
       }
       for (int q = 0; q < p.nc; ++q) kc.push_back(kpt[q]);
+      // y-independent part of the tridiagonal sweep of the k3 spline
+      {
+        const double *x = kpt;
+        std::vector<double> sig(p.nc, 0.0), pv(p.nc, 0.0), gam(p.nc, 0.0);
+        gam[0] = -0.5;
+        for (int i = 1; i < p.nc - 1; ++i) {
+          sig[i] = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+          pv[i] = sig[i] * gam[i - 1] + 2.0;
+          gam[i] = (sig[i] - 1.0) / pv[i];
+        }
+        spl.insert(spl.end(), sig.begin(), sig.end());
+        spl.insert(spl.end(), pv.begin(), pv.end());
+        spl.insert(spl.end(), gam.begin(), gam.end());
+      }
+      e->nc_max = std::max(e->nc_max, p.nc);
       for (int it = 0; it < p.nf; ++it) {
         const int s = it < first ? first : (it > last ? last : it);
         kf.push_back(klist[it]);
@@ -379,6 +405,10 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
       e->pairs.push_back(p);
     }
   e->F = (int)kf.size();
+  // processing order: fine k3 sorted by value over all pairs (ties by index)
+  std::vector<int> forder(e->F);
+  for (int i = 0; i < e->F; ++i) forder[i] = i;
+  std::stable_sort(forder.begin(), forder.end(), [&](int a, int b) { return kf[a] < kf[b]; });
   e->src_total = src_off;
   e->tt2_size = ptr2->tt2_size;
   e->ntau = ntau;
@@ -393,17 +423,22 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
 
   if (upload_vec(e, e->dt0mt, t0mt, err) || upload_vec(e, e->ddtau, dtau, err) ||
       upload_vec(e, e->dpairs, e->pairs, err) || upload_vec(e, e->dkc, kc, err) ||
-      upload_vec(e, e->dkf, kf, err) || upload_vec(e, e->dfpair, fpair, err) ||
-      upload_vec(e, e->dfsrc, fsrc, err) || upload_vec(e, e->dfbr, fbr, err))
+      upload_vec(e, e->dkf, kf, err) || upload_vec(e, e->dspl, spl, err) ||
+      upload_vec(e, e->dfpair, fpair, err) ||
+      upload_vec(e, e->dfsrc, fsrc, err) || upload_vec(e, e->dfbr, fbr, err) ||
+      upload_vec(e, e->dforder, forder, err))
     return _FAILURE_;
 
-  // sources: per pair, tp2 contiguous [tau][kc] blocks
-  HIPCHK(hipMalloc((void **)&e->dsrc, std::max<long>(1, e->src_total) * sizeof(double)), err);
-  HIPCHK(hipMalloc((void **)&e->ddd, std::max<long>(1, e->src_total) * sizeof(double)), err);
+  // sources as SONG holds them: per pair, tp2 contiguous [tau][kc] blocks; the
+  // run re-lays them out to [kc][tau] (dsrc) and splines them (ddd)
+  const size_t sbytes = std::max<long>(1, e->src_total) * sizeof(double);
+  HIPCHK(hipMalloc((void **)&e->draw, sbytes), err);
+  HIPCHK(hipMalloc((void **)&e->dsrc, sbytes), err);
+  HIPCHK(hipMalloc((void **)&e->ddd, sbytes), err);
   for (const los_pair &p : e->pairs)
     for (int tp = 0; tp < tp2; ++tp) {
       const size_t n = (size_t)ntau * p.nc;
-      if (upload(e, e->dsrc + p.src_off + (long)tp * n, ppt2->sources[tp][p.k1][p.k2],
+      if (upload(e, e->draw + p.src_off + (long)tp * n, ppt2->sources[tp][p.k1][p.k2],
                  n * sizeof(double), err))
         return _FAILURE_;
     }
@@ -419,6 +454,8 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
   P.fpair = e->dfpair;
   P.fsrc = e->dfsrc;
   P.fbr = e->dfbr;
+  P.forder = e->dforder;
+  P.f_per_xcd = (e->F + 7) / 8;
   P.src = e->dsrc;
   P.dd = e->ddd;
   P.F = e->F;
@@ -477,14 +514,34 @@ extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
   }
   HIPCHK(hipSetDevice(e->device), err);
   const int tp2 = e->P.tp2_size;
+  const int np = (int)e->pairs.size();
   HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
+  HIPCHK(los_launch_transpose(e->dpairs, np, e->nc_max, e->draw, e->dsrc, e->ntau, tp2, e->stream),
+         err);
   if (e->P.cubic_k3)
-    HIPCHK(los_launch_spline_exact(e->dpairs, (int)e->pairs.size(), e->dkc, e->dsrc, e->ddd,
-                                   e->ntau, tp2, e->stream),
+    HIPCHK(los_launch_spline_tau(e->dpairs, np, e->dkc, e->dspl, e->dsrc, e->ddd, e->ntau, tp2,
+                                 e->stream),
            err);
   HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
-  const int threads = ((e->n_lm + 63) / 64) * 64;
-  HIPCHK(los_launch_integrate_exact(&e->P, threads, e->stream), err);
+  if (e->form == LOS_FORM_FP32C && e->dJ32 == nullptr) {
+    // FP32 copy of the resident J table, made once per table upload
+    HIPCHK(hipMalloc((void **)&e->dJ32, std::max<size_t>(1, e->nJ) * sizeof(float)), err);
+    HIPCHK(los_launch_round_j32(e->dJ, e->dJ32, (long)e->nJ, e->stream), err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+    HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
+  }
+  e->P.J32 = e->dJ32;
+  switch (e->form) {
+    case LOS_FORM_FP64:
+      HIPCHK(los_launch_integrate_exact(&e->P, 0, e->stream), err);
+      break;
+    case LOS_FORM_FP64_FAST:
+      HIPCHK(los_launch_stream_fp64(&e->P, e->stream), err);
+      break;
+    default:
+      HIPCHK(los_launch_stream_fp32c(&e->P, e->stream), err);
+      break;
+  }
   HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
   HIPCHK(hipEventSynchronize(e->ev[2]), err);
   HIPCHK(hipEventElapsedTime(&e->ms_spline, e->ev[0], e->ev[1]), err);

@@ -14,58 +14,93 @@
 
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "los_device.h"
 
 extern "C" {
 
 // ---------------------------------------------------------------------------
-// Spline second derivatives in k3, one thread per (pair, tp, tau) column.
-// Column layout [tau][kc] inside each (pair, tp) block: column c of pair p
-// starts at src_off + (tp*ntau + tau)*nc and is contiguous in kc.
-// End slopes from the parabola through the three end points.
+// Source re-layout: each (pair, tp) block [tau][kc] (k3 fastest, as SONG
+// stores it) becomes [kc][tau], so that the lanes of a wave, which run over
+// tau, read contiguous memory. 32x32 tiles through LDS.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-los_spline_k3_exact(const los_pair *pairs, long ncols, const double *kc_all, const double *src,
-                    double *dd, int ntau, int tp2_size)
+los_transpose_sources(const los_pair *pairs, const double *raw, double *out, int ntau, int tp2)
+{
+  __shared__ double tile[32][33];
+  const int mat = blockIdx.z;  // pair * tp2 + tp
+  const int p = mat / tp2, tp = mat - p * tp2;
+  const int nc = pairs[p].nc;
+  const int kc0 = blockIdx.x * 32, t0 = blockIdx.y * 32;
+  if (kc0 >= nc) return;
+  const long base = pairs[p].src_off + (long)tp * ntau * nc;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int t = t0 + r, kc = kc0 + tx;
+    if (t < ntau && kc < nc) tile[r][tx] = raw[base + (long)t * nc + kc];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int kc = kc0 + r, t = t0 + tx;
+    if (t < ntau && kc < nc) out[base + (long)kc * ntau + t] = tile[tx][r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spline second derivatives in k3 on the tau-contiguous layout: one thread
+// per (pair, tp, tau) column, consecutive threads on consecutive tau. The
+// y-independent recurrence values sig_i, p_i and gamma_i = (sig_i-1)/p_i of the
+// tridiagonal sweep are computed on the host with the same operations and
+// passed per pair (spl[3*kc_off + {0,1,2}*nc + i]); u_i is kept in dd until
+// the back substitution overwrites it.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+los_spline_k3_tau(const los_pair *pairs, long ncols, const double *kc_all,
+                  const double *spl, const double *src, double *dd, int ntau, int tp2_size)
 {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncols) return;
-  // every pair owns tp2_size*ntau columns; column r of a pair is (tp*ntau + tau)
   const long per_pair = (long)tp2_size * ntau;
   const int p = (int)(c / per_pair);
   const long r = c - (long)p * per_pair;
+  const int tp = (int)(r / ntau);
+  const int tau = (int)(r - (long)tp * ntau);
   const int n = pairs[p].nc;
   const double *x = kc_all + pairs[p].kc_off;
-  const long off = pairs[p].src_off + r * n;
-  const double *y = src + off;
-  double *d = dd + off;
-  double u[LOS_NC_MAX];
+  const double *sig = spl + 3L * pairs[p].kc_off;
+  const double *pv = sig + n;
+  const double *gam = pv + n;
+  const long base = pairs[p].src_off + (long)tp * n * ntau + tau;
+  const double *y = src + base;
+  double *d = dd + base;
+  const long s = ntau;
 
-  double d0 = ((x[2] - x[0]) * (x[2] - x[0]) * (y[1] - y[0]) -
-               (x[1] - x[0]) * (x[1] - x[0]) * (y[2] - y[0])) /
+  double y0 = y[0], y1 = y[s], y2 = y[2 * s];
+  double d0 = ((x[2] - x[0]) * (x[2] - x[0]) * (y1 - y0) - (x[1] - x[0]) * (x[1] - x[0]) * (y2 - y0)) /
               ((x[2] - x[0]) * (x[1] - x[0]) * (x[2] - x[1]));
-  double ddprev = -0.5;
-  d[0] = ddprev;
-  u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - d0);
+  double u = (3.0 / (x[1] - x[0])) * ((y1 - y0) / (x[1] - x[0]) - d0);
+  d[0] = u;
+  double ym = y0, yc = y1;
   for (int i = 1; i < n - 1; ++i) {
-    double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
-    double pp = sig * ddprev + 2.0;
-    ddprev = (sig - 1.0) / pp;
-    d[i] = ddprev;
-    double t = (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
-    u[i] = (6.0 * t / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / pp;
+    const double yn = y[(long)(i + 1) * s];
+    double t = (yn - yc) / (x[i + 1] - x[i]) - (yc - ym) / (x[i] - x[i - 1]);
+    u = (6.0 * t / (x[i + 1] - x[i - 1]) - sig[i] * u) / pv[i];
+    d[(long)i * s] = u;
+    ym = yc;
+    yc = yn;
   }
   const int a = n - 3, b = n - 2, e = n - 1;
-  double dn = ((x[e] - x[a]) * (x[e] - x[a]) * (y[e] - y[b]) -
-               (x[e] - x[b]) * (x[e] - x[b]) * (y[e] - y[a])) /
+  const double ya = y[(long)a * s], yb = y[(long)b * s], ye = y[(long)e * s];
+  double dn = ((x[e] - x[a]) * (x[e] - x[a]) * (ye - yb) - (x[e] - x[b]) * (x[e] - x[b]) * (ye - ya)) /
               ((x[e] - x[a]) * (x[e] - x[b]) * (x[b] - x[a]));
   double qn = 0.5;
-  double un = (3.0 / (x[e] - x[b])) * (dn - (y[e] - y[b]) / (x[e] - x[b]));
-  double next = (un - qn * u[n - 2]) / (qn * ddprev + 1.0);
-  d[n - 1] = next;
+  double un = (3.0 / (x[e] - x[b])) * (dn - (ye - yb) / (x[e] - x[b]));
+  double next = (un - qn * u) / (qn * gam[n - 2] + 1.0);
+  d[(long)e * s] = next;
   for (int k = n - 2; k >= 0; --k) {
-    next = d[k] * next + u[k];
-    d[k] = next;
+    next = gam[k] * next + d[(long)k * s];
+    d[(long)k * s] = next;
   }
 }
 
@@ -80,7 +115,16 @@ __device__ __forceinline__ int index_x_of(double k, double t0mt, double xx_step)
   return (int)(x / xx_step);
 }
 
-__global__ void __launch_bounds__(1024)
+}  // extern "C"
+
+// NL = number of source multipoles L handled (max(l_max_los_t, l_max_los_p)+1).
+// All 2*3*NL J loads of a time step are issued before any of them is used, so
+// a wave keeps many independent loads in flight; the reference's skips are
+// then applied as selects on the summation, which leaves the per-element
+// operation order untouched. Rows that the reference skips still have valid
+// addresses (the table is zero-filled below each row's first live index).
+template <int NL>
+__global__ void __launch_bounds__(LOS_EXACT_BLOCK_MAX)
 los_integrate_exact(los_params P)
 {
   extern __shared__ double smem[];
@@ -89,7 +133,11 @@ los_integrate_exact(los_params P)
   int *sIX = (int *)(sA + CH);                        // index_x[CH]
   __shared__ int s_tau_end;
 
-  const int f = blockIdx.x;
+  // workgroups are dispatched round-robin over the 8 XCDs: give XCD (b % 8)
+  // the sorted positions [(b % 8) * f_per_xcd, ...) in order
+  const int spos = (blockIdx.x & 7) * P.f_per_xcd + (blockIdx.x >> 3);
+  if (spos >= P.F) return;
+  const int f = P.forder[spos];
   const int pidx = P.fpair[f];
   const los_pair pr = P.pairs[pidx];
   const double k = P.kf[f];
@@ -103,7 +151,7 @@ los_integrate_exact(los_params P)
   const double ca = aw * aw * aw - aw;
   const double cb = bw * bw * bw - bw;
 
-  const int lm = threadIdx.x;
+  const int lm = blockIdx.y * blockDim.x + threadIdx.x;
   const bool active = lm < P.n_lm;
 
   // Live tau range of this lane: index_x falls with tau, so the terms with
@@ -130,7 +178,39 @@ los_integrate_exact(los_params P)
     m = P.m_of[lm];
     mi = P.mi_of[lm];
   }
+  // Per-lane row constants, hoisted out of the tau loop: first live x index of
+  // each (J type, L) row (INT_MAX for rows the reference skips: m > L, EB at
+  // m = 0), the row's base offset in J, and the LDS row of its source.
+  int xmT[NL], xmE[NL], xmM[NL];
+  int sT[NL], sE[NL], sB[NL];
+  int rowT[NL], rowE[NL], rowM[NL];  // (jt*L_size + L) * X
+#pragma unroll
+  for (int L = 0; L < NL; ++L) {
+    xmT[L] = xmE[L] = xmM[L] = INT_MAX;
+    sT[L] = sE[L] = sB[L] = 0;
+    rowT[L] = rowE[L] = rowM[L] = 0;
+    if (!active || m > L || L >= P.L_size) continue;
+    if (P.has_T && L <= P.LmaxT) {
+      const long row = (long)P.jTT * P.L_size + L;
+      xmT[L] = P.xmin[row * P.n_lm + lm];
+      rowT[L] = (int)row * P.X;
+      sT[L] = P.srcT[L][mi] * CH;
+    }
+    if ((P.has_E || P.has_B) && L <= P.LmaxP) {
+      const long rE = (long)P.jEE * P.L_size + L;
+      xmE[L] = P.xmin[rE * P.n_lm + lm];
+      rowE[L] = (int)rE * P.X;
+      if (P.has_E) sE[L] = P.srcE[L][mi] * CH;
+      if (P.has_B) sB[L] = P.srcB[L][mi] * CH;
+      if (P.jEB >= 0 && P.has_B) {
+        const long rM = (long)P.jEB * P.L_size + L;
+        xmM[L] = P.xmin[rM * P.n_lm + lm];
+        rowM[L] = (int)rM * P.X;
+      }
+    }
+  }
 
+  const double *Jl = P.J + lm;
   double accT = 0, accEd = 0, accEm = 0, accBd = 0, accBm = 0;
 
   for (int c0 = 0; c0 < tau_end_block; c0 += CH) {
@@ -140,14 +220,15 @@ los_integrate_exact(los_params P)
     for (int idx = threadIdx.x; idx < P.tp2_size * CH; idx += blockDim.x) {
       const int tp = idx / CH, c = idx - tp * CH;
       if (c >= nch) continue;
-      const long base = pr.src_off + ((long)tp * P.ntau + (c0 + c)) * nc;
+      // tau-contiguous layout [tp][kc][tau]
+      const long base = pr.src_off + ((long)tp * nc + br) * P.ntau + (c0 + c);
       const double *S = P.src + base;
       double v;
       if (P.cubic_k3) {
         const double *D = P.dd + base;
-        v = aw * S[br] + bw * S[br + 1] + (ca * D[br] + cb * D[br + 1]) * h * h / 6.0;
+        v = aw * S[0] + bw * S[P.ntau] + (ca * D[0] + cb * D[P.ntau]) * h * h / 6.0;
       } else {
-        v = aw * S[br] + bw * S[br + 1];
+        v = aw * S[0] + bw * S[P.ntau];
       }
       sS[idx] = v;
     }
@@ -164,37 +245,53 @@ los_integrate_exact(los_params P)
       const int ix = sIX[c];
       const double aJ = sA[c];
       const double dt = P.dtau[c0 + c];
+      // loads of live rows only (exec-masked: dead lanes fetch nothing)
+      double JT0[NL], JT1[NL], JE0[NL], JE1[NL], JM0[NL], JM1[NL];
+#pragma unroll
+      for (int L = 0; L < NL; ++L) {
+        JT0[L] = JT1[L] = JE0[L] = JE1[L] = JM0[L] = JM1[L] = 0.0;
+        if (ix >= xmT[L]) {
+          const double *a = Jl + (long)(rowT[L] + ix) * P.n_lm;
+          JT0[L] = a[0];
+          JT1[L] = a[P.n_lm];
+        }
+        if (ix >= xmE[L]) {
+          const double *b = Jl + (long)(rowE[L] + ix) * P.n_lm;
+          JE0[L] = b[0];
+          JE1[L] = b[P.n_lm];
+        }
+        if (ix >= xmM[L]) {
+          const double *d = Jl + (long)(rowM[L] + ix) * P.n_lm;
+          JM0[L] = d[0];
+          JM1[L] = d[P.n_lm];
+        }
+      }
       if (P.has_T) {
         double integrand = 0;
-        for (int L = 0; L <= P.LmaxT; ++L) {
-          if (m > L) continue;
-          const long row = (long)P.jTT * P.L_size + L;
-          if (ix < P.xmin[row * P.n_lm + lm]) continue;
-          const double *J = P.J + (row * P.X + ix) * P.n_lm + lm;
-          const double Jv = aJ * J[0] + (1 - aJ) * J[P.n_lm];
-          integrand += Jv * sS[P.srcT[L][mi] * CH + c];
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          if (ix >= xmT[L]) {  // xmT = INT_MAX for rows the reference skips
+            const double Jv = aJ * JT0[L] + (1 - aJ) * JT1[L];
+            integrand += Jv * sS[sT[L] + c];
+          }
         }
         accT += integrand * dt;
       }
       if (P.has_E || P.has_B) {
         double iEd = 0, iEm = 0, iBd = 0, iBm = 0;
-        for (int L = 0; L <= P.LmaxP; ++L) {
-          if (m > L) continue;
-          const long rEE = (long)P.jEE * P.L_size + L;
-          if (ix >= P.xmin[rEE * P.n_lm + lm]) {
-            const double *J = P.J + (rEE * P.X + ix) * P.n_lm + lm;
-            const double Jv = aJ * J[0] + (1 - aJ) * J[P.n_lm];
-            if (P.has_E) iEd += Jv * sS[P.srcE[L][mi] * CH + c];
-            if (P.has_B) iBd += Jv * sS[P.srcB[L][mi] * CH + c];
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          if (ix >= xmE[L]) {
+            const double Jv = aJ * JE0[L] + (1 - aJ) * JE1[L];
+            if (P.has_E) iEd += Jv * sS[sE[L] + c];
+            if (P.has_B) iBd += Jv * sS[sB[L] + c];
           }
-          if (P.jEB >= 0 && m != 0) {
-            const long rEB = (long)P.jEB * P.L_size + L;
-            if (ix >= P.xmin[rEB * P.n_lm + lm]) {
-              const double *J = P.J + (rEB * P.X + ix) * P.n_lm + lm;
-              const double Jv = aJ * J[0] + (1 - aJ) * J[P.n_lm];
-              if (P.has_E && P.has_B) iEm += Jv * sS[P.srcB[L][mi] * CH + c];
-              if (P.has_B) iBm += Jv * sS[P.srcE[L][mi] * CH + c];
-            }
+          if (ix >= xmM[L]) {
+            const double Jv = aJ * JM0[L] + (1 - aJ) * JM1[L];
+            // E adds the B->E mixing only for m != 0 (source/transfer2.c:1961);
+            // B always subtracts the E->B mixing (source/transfer2.c:2034-2059)
+            if (m != 0) iEm += Jv * sS[sB[L] + c];
+            iBm += Jv * sS[sE[L] + c];
           }
         }
         accEd += iEd * dt;
@@ -244,25 +341,65 @@ los_integrate_exact(los_params P)
 // ---------------------------------------------------------------------------
 // Launchers (host).
 // ---------------------------------------------------------------------------
-hipError_t los_launch_spline_exact(const los_pair *pairs, int npairs, const double *kc,
-                                   const double *src, double *dd, int ntau, int tp2,
-                                   hipStream_t s)
+template <int NL>
+static hipError_t launch_exact_nl(const los_params *P, dim3 grid, int threads, size_t shmem,
+                                  hipStream_t s)
+{
+  if (shmem > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)los_integrate_exact<NL>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(los_integrate_exact<NL>, grid, dim3(threads), shmem, s, *P);
+  return hipGetLastError();
+}
+
+extern "C" {
+
+hipError_t los_launch_transpose(const los_pair *pairs, int npairs, int nc_max, const double *raw,
+                                double *out, int ntau, int tp2, hipStream_t s)
+{
+  if (npairs == 0) return hipSuccess;
+  dim3 grid((nc_max + 31) / 32, (ntau + 31) / 32, (unsigned)(npairs * tp2));
+  hipLaunchKernelGGL(los_transpose_sources, grid, dim3(256), 0, s, pairs, raw, out, ntau, tp2);
+  return hipGetLastError();
+}
+
+hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double *kc,
+                                 const double *spl, const double *src, double *dd, int ntau,
+                                 int tp2, hipStream_t s)
 {
   const long ncols = (long)npairs * tp2 * ntau;
   if (ncols == 0) return hipSuccess;
   const int block = 256;
   const long grid = (ncols + block - 1) / block;
-  hipLaunchKernelGGL(los_spline_k3_exact, dim3((unsigned)grid), dim3(block), 0, s, pairs, ncols,
-                     kc, src, dd, ntau, tp2);
+  hipLaunchKernelGGL(los_spline_k3_tau, dim3((unsigned)grid), dim3(block), 0, s, pairs, ncols, kc,
+                     spl, src, dd, ntau, tp2);
   return hipGetLastError();
 }
 
 hipError_t los_launch_integrate_exact(const los_params *P, int nthreads, hipStream_t s)
 {
-  if (P->F == 0) return hipSuccess;
+  (void)nthreads;
+  if (P->F == 0 || P->n_lm == 0) return hipSuccess;
+  // (l,m) lanes split over gridDim.y blocks of at most LOS_EXACT_BLOCK_MAX lanes
+  const int nblk = (P->n_lm + LOS_EXACT_BLOCK_MAX - 1) / LOS_EXACT_BLOCK_MAX;
+  const int per = (P->n_lm + nblk - 1) / nblk;
+  const int threads = ((per + 63) / 64) * 64;
   const size_t shmem = (size_t)P->tp2_size * CH * sizeof(double) + CH * (sizeof(double) + sizeof(int));
-  hipLaunchKernelGGL(los_integrate_exact, dim3(P->F), dim3(nthreads), shmem, s, *P);
-  return hipGetLastError();
+  const int NLr = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
+  const dim3 grid(8 * P->f_per_xcd, nblk);
+  switch (NLr) {
+    case 1: return launch_exact_nl<1>(P, grid, threads, shmem, s);
+    case 2: return launch_exact_nl<2>(P, grid, threads, shmem, s);
+    case 3: return launch_exact_nl<3>(P, grid, threads, shmem, s);
+    case 4: return launch_exact_nl<4>(P, grid, threads, shmem, s);
+    case 5: return launch_exact_nl<5>(P, grid, threads, shmem, s);
+    case 6: return launch_exact_nl<6>(P, grid, threads, shmem, s);
+    case 7: return launch_exact_nl<7>(P, grid, threads, shmem, s);
+    case 8: return launch_exact_nl<8>(P, grid, threads, shmem, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // extern "C"

@@ -390,7 +390,9 @@ int los_scenario_gather(los_scenario *s, double *out, long n, char *err)
   long o = 0;
   for (int tt = 0; tt < ptr2->tt2_size; ++tt)
     for (int i1 = 0; i1 < ppt2->k_size; ++i1) {
-      los_test(!ptr2->has_allocated_transfers[i1], err, "k1 level %d not allocated", i1);
+      /* allocated per k1 (transfer2_init) or per type (load from disk) */
+      los_test(ptr2->transfer[tt] == NULL || ptr2->transfer[tt][i1] == NULL, err,
+               "transfer level tt=%d k1=%d not allocated", tt, i1);
       for (int i2 = 0; i2 <= i1; ++i2) {
         int nk = ptr2->k_size_k1k2[i1][i2];
         memcpy(out + o, ptr2->transfer[tt][i1][i2], nk * sizeof(double));
@@ -448,6 +450,92 @@ double *los_scenario_J_row(los_scenario *s, int J, int L, int il, int im, int *x
   *xmin = b->index_xmin_J[J][L][il][im];
   *xsize = b->x_size_J[J][L][il][im];
   return b->J_Llm_x[J][L][il][im];
+}
+
+/* Copies of the generated grids (l list, k grid, tau sampling). */
+int los_scenario_grids(los_scenario *s, int *l_out, double *k_out, double *tau_out)
+{
+  if (l_out) memcpy(l_out, s->bs.l, s->bs.l_size * sizeof(int));
+  if (k_out) memcpy(k_out, s->pt2.k, s->pt2.k_size * sizeof(double));
+  if (tau_out) memcpy(tau_out, s->pt2.tau_sampling, s->pt2.tau_size * sizeof(double));
+  return _SUCCESS_;
+}
+
+/* Replace the (k1,k2) grid values (same size). */
+int los_scenario_override_k(los_scenario *s, const double *k, int n, char *err)
+{
+  los_test(n != s->pt2.k_size, err, "k grid has %d values, got %d", s->pt2.k_size, n);
+  memcpy(s->pt2.k, k, n * sizeof(double));
+  return _SUCCESS_;
+}
+
+/* Replace the coarse k3 grid of one pair (any size >= 2); the pair's sources
+   are reallocated (zero) with the new size. */
+int los_scenario_override_k3(los_scenario *s, int i1, int i2, const double *k3, int n, char *err)
+{
+  struct perturbs2 *ppt2 = &s->pt2;
+  los_test(i2 > i1 || i1 >= ppt2->k_size || n < 2, err, "bad pair or size");
+  free(ppt2->k3[i1][i2]);
+  los_alloc(ppt2->k3[i1][i2], n * sizeof(double), err);
+  memcpy(ppt2->k3[i1][i2], k3, n * sizeof(double));
+  ppt2->k3_size[i1][i2] = n;
+  for (int tp = 0; tp < ppt2->tp2_size; ++tp) {
+    free(ppt2->sources[tp][i1][i2]);
+    los_calloc(ppt2->sources[tp][i1][i2], (size_t)n * ppt2->tau_size, sizeof(double), err);
+  }
+  return _SUCCESS_;
+}
+
+/* Replace one J row: it is stored from index xmin with n values. */
+int los_scenario_override_J(los_scenario *s, int J, int L, int il, int im, int xmin,
+                            const double *v, int n, char *err)
+{
+  struct bessels2 *b = &s->bs2;
+  los_test(xmin < 0 || xmin + n != b->xx_size, err, "row must cover [xmin, xx_size)");
+  free(b->J_Llm_x[J][L][il][im]);
+  los_alloc(b->J_Llm_x[J][L][il][im], n * sizeof(double), err);
+  memcpy(b->J_Llm_x[J][L][il][im], v, n * sizeof(double));
+  b->index_xmin_J[J][L][il][im] = xmin;
+  b->x_size_J[J][L][il][im] = n;
+  b->x_min_J[J][L][il][im] = b->xx[xmin];
+  return _SUCCESS_;
+}
+
+/* Inverse of los_scenario_gather: write values into ptr2->transfer. */
+int los_scenario_scatter(los_scenario *s, const double *in, long n, char *err)
+{
+  struct transfers2 *ptr2 = &s->tr2;
+  struct perturbs2 *ppt2 = &s->pt2;
+  long need = los_scenario_n_transfers(s);
+  los_test(n < need, err, "buffer of %ld values, need %ld", n, need);
+  long o = 0;
+  for (int tt = 0; tt < ptr2->tt2_size; ++tt)
+    for (int i1 = 0; i1 < ppt2->k_size; ++i1) {
+      los_test(!ptr2->has_allocated_transfers[i1], err, "k1 level %d not allocated", i1);
+      for (int i2 = 0; i2 <= i1; ++i2) {
+        int nk = ptr2->k_size_k1k2[i1][i2];
+        memcpy(ptr2->transfer[tt][i1][i2], in + o, nk * sizeof(double));
+        o += nk;
+      }
+    }
+  return _SUCCESS_;
+}
+
+/* Bookkeeping readout: sizes per pair (k_size, physical start, physical size). */
+int los_scenario_k3_sizes(los_scenario *s, int *out, long n, char *err)
+{
+  struct transfers2 *ptr2 = &s->tr2;
+  struct perturbs2 *ppt2 = &s->pt2;
+  long need = 3L * ppt2->k_size * (ppt2->k_size + 1) / 2;
+  los_test(n < need || ptr2->k_size_k1k2 == NULL, err, "need %ld ints after the indices", need);
+  long o = 0;
+  for (int i1 = 0; i1 < ppt2->k_size; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2) {
+      out[o++] = ptr2->k_size_k1k2[i1][i2];
+      out[o++] = ptr2->k_physical_start_k1k2[i1][i2];
+      out[o++] = ptr2->k_physical_size_k1k2[i1][i2];
+    }
+  return _SUCCESS_;
 }
 
 void los_scenario_destroy(los_scenario *s)

@@ -362,7 +362,7 @@ int transfer2_indices_of_transfers(struct precision *ppr, struct precision2 *ppr
   ptr2->count_allocated_transfers = 0;
   los_alloc(ptr2->transfer, ptr2->tt2_size * sizeof(double ***), ptr2->error_message);
   for (int tt = 0; tt < ptr2->tt2_size; ++tt)
-    los_alloc(ptr2->transfer[tt], ppt2->k_size * sizeof(double **), ptr2->error_message);
+    los_calloc(ptr2->transfer[tt], ppt2->k_size, sizeof(double **), ptr2->error_message);
   los_calloc(ptr2->has_allocated_transfers, ppt2->k_size, sizeof(short), ptr2->error_message);
 
   if (ppr2->store_transfers_to_disk || ppr2->load_transfers_from_disk) {
@@ -434,6 +434,7 @@ int transfer2_free_k1_level(struct perturbs2 *ppt2, struct transfers2 *ptr2, int
   for (int tt = 0; tt < ptr2->tt2_size; tt++) {
     for (int i2 = 0; i2 <= index_k1; ++i2) free(ptr2->transfer[tt][index_k1][i2]);
     free(ptr2->transfer[tt][index_k1]);
+    ptr2->transfer[tt][index_k1] = NULL;
   }
   ptr2->has_allocated_transfers[index_k1] = _FALSE_;
   return _SUCCESS_;

@@ -71,6 +71,13 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_scenario_set_string": (i, [vp, ch, ch, ch]),
         "los_scenario_source_ptr": (C.POINTER(d), [vp, i, i, i]),
         "los_scenario_J_row": (C.POINTER(d), [vp, i, i, i, i, C.POINTER(i), C.POINTER(i)]),
+        "los_scenario_grids": (i, [vp, C.POINTER(i), C.POINTER(d), C.POINTER(d)]),
+        "los_scenario_override_k": (i, [vp, C.POINTER(d), i, ch]),
+        "los_scenario_override_k3": (i, [vp, i, i, C.POINTER(d), i, ch]),
+        "los_scenario_override_J": (i, [vp, i, i, i, i, i, C.POINTER(d), i, ch]),
+        "los_scenario_scatter": (i, [vp, C.POINTER(d), C.c_long, ch]),
+        "los_scenario_k3_sizes": (i, [vp, C.POINTER(i), C.c_long, ch]),
+        "los_bessel2_build": (i, [vp] * 5 + [ch]),
         "los_sph_bessel_array": (None, [i, d, C.POINTER(d)]),
         "los_wigner3j_j1": (i, [i, i, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(d), i]),
         "transfer2_init": (i, [vp] * 10),
@@ -109,7 +116,7 @@ def load() -> C.CDLL:
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise LosError(
-                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                f"{LIB_PATH} is missing: build it with `python -m song_amd.build` "
                 "(the LOS stage has no fallback implementation)")
         _LIB = _declare(C.CDLL(LIB_PATH))
     return _LIB
@@ -200,6 +207,7 @@ class Scenario:
         self.lib.los_scenario_structs(self.ptr, *[C.byref(p) for p in ptrs])
         self.s = dict(zip(STRUCT_NAMES, ptrs))
         self._inited = False
+        self._k3_counts = {}
 
     def structs(self):
         return [self.s[n] for n in STRUCT_NAMES]
@@ -232,6 +240,68 @@ class Scenario:
         check(self.lib.los_scenario_gather(self.ptr, out.ctypes.data_as(C.POINTER(C.c_double)),
                                            n, err), err, "gather")
         return out
+
+    def grids(self):
+        """(l list, k grid, tau sampling) as generated."""
+        ls = np.zeros(self.cfg.l_size, dtype=np.int32)
+        ks = np.zeros(self.cfg.k_size)
+        ts = np.zeros(self.cfg.tau_size)
+        self.lib.los_scenario_grids(self.ptr, ls.ctypes.data_as(C.POINTER(C.c_int)),
+                                    ks.ctypes.data_as(C.POINTER(C.c_double)),
+                                    ts.ctypes.data_as(C.POINTER(C.c_double)))
+        return ls, ks, ts
+
+    def scatter(self, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        err = _err()
+        check(self.lib.los_scenario_scatter(self.ptr, v.ctypes.data_as(C.POINTER(C.c_double)),
+                                            v.size, err), err, "scatter")
+
+    def k3_sizes(self) -> np.ndarray:
+        """(k_size, physical start, physical size) per pair, after the indices."""
+        nk = self.cfg.k_size
+        out = np.zeros(3 * nk * (nk + 1) // 2, dtype=np.int32)
+        err = _err()
+        check(self.lib.los_scenario_k3_sizes(self.ptr, out.ctypes.data_as(C.POINTER(C.c_int)),
+                                             out.size, err), err, "k3_sizes")
+        return out.reshape(-1, 3)
+
+    def override_k(self, k) -> None:
+        v = np.ascontiguousarray(k, dtype=np.float64)
+        err = _err()
+        check(self.lib.los_scenario_override_k(self.ptr, v.ctypes.data_as(C.POINTER(C.c_double)),
+                                               v.size, err), err, "override_k")
+
+    def override_k3(self, i1: int, i2: int, k3) -> None:
+        v = np.ascontiguousarray(k3, dtype=np.float64)
+        err = _err()
+        check(self.lib.los_scenario_override_k3(self.ptr, i1, i2,
+                                                v.ctypes.data_as(C.POINTER(C.c_double)),
+                                                v.size, err), err, "override_k3")
+        self._k3_counts[(i1, i2)] = v.size
+
+    def override_J(self, jt: int, L: int, il: int, im: int, xmin: int, values) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        err = _err()
+        check(self.lib.los_scenario_override_J(self.ptr, jt, L, il, im, xmin,
+                                               v.ctypes.data_as(C.POINTER(C.c_double)), v.size,
+                                               err), err, "override_J")
+
+    def source(self, tp: int, i1: int, i2: int) -> np.ndarray:
+        """Writable view of ppt2->sources[tp][i1][i2] as [tau][k3]."""
+        p = self.lib.los_scenario_source_ptr(self.ptr, tp, i1, i2)
+        if not p:
+            raise LosError("sources not allocated")
+        nk3 = self.k3_count(i1, i2)
+        return np.ctypeslib.as_array(p, shape=(self.cfg.tau_size, nk3))
+
+    def k3_count(self, i1: int, i2: int) -> int:
+        return self._k3_counts.get((i1, i2), self.cfg.k3_size)
+
+    def J_row(self, jt: int, L: int, il: int, im: int):
+        xmin, xs = C.c_int(), C.c_int()
+        p = self.lib.los_scenario_J_row(self.ptr, jt, L, il, im, C.byref(xmin), C.byref(xs))
+        return xmin.value, np.ctypeslib.as_array(p, shape=(xs.value,))
 
     def free_transfers(self, fn=None) -> None:
         if self._inited:

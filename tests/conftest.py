@@ -18,7 +18,7 @@ ORACLE_PATH = os.path.join(ROOT, "oracle", "liblos_oracle.so")
 
 
 def load_oracle():
-    """The CPU oracle library (test infrastructure; built by __graft_entry__.build())."""
+    """The CPU oracle library (test infrastructure; built by `python -m song_amd.build`)."""
     if not os.path.exists(ORACLE_PATH):
         pytest.skip("oracle library not built")
     o = C.CDLL(ORACLE_PATH)

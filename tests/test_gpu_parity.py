@@ -46,3 +46,75 @@ def test_fp64_bit_exact(name, oracle):
                             f"{got[diff[:5]]} vs {ref[diff[:5]]}")
     gpu_s.close()
     ref_s.close()
+
+
+@pytest.mark.gpu
+def test_closed_form_gpu():
+    """J linear in x, sources constant: the library's transfers equal the closed form."""
+    from tests.test_oracle_pinning import CF_CFG, closed_form_case, expected_closed_form, \
+        _product_k3_list
+    s, cfg, fields, tp_of, pair_c, I, lvals = closed_form_case(CF_CFG)
+    s.transfer2_init()
+    got = s.gather()
+    sizes = s.k3_sizes()
+    lists, q = {}, 0
+    for i1 in range(cfg.k_size):
+        for i2 in range(i1 + 1):
+            lists[(i1, i2)] = _product_k3_list(s, i1, i2, int(sizes[q, 0]))
+            q += 1
+    exp = expected_closed_form(s, cfg, fields, pair_c, I, lvals, lists)
+    assert got.shape == exp.shape
+    assert np.max(np.abs(got - exp)) < 1e-11 * np.max(np.abs(exp))
+    s.close()
+
+
+@pytest.mark.gpu
+def test_fp64_bit_exact_wide_grid(oracle):
+    """c2-like grid (log k from 1e-5 to 0.1, T+E+B, m=0,1,2, L=0..6) at reduced size."""
+    cfg = L.Config(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=6, l_max_los_p=6, k_size=8,
+                   k_min=1e-5, k_max=0.1, k3_size=24, tau_size=200, l_size=24, l_max=2000,
+                   dk_fine=6e-4, xx_max=1500.0)
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    s = L.Scenario(cfg)
+    s.transfer2_init()
+    got = s.gather()
+    s.close()
+    assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} of {ref.size} differ"
+
+
+def _row_errors(got, ref, scen):
+    sizes = [int(r[0]) for r in scen.k3_sizes()]
+    tt2 = got.size // sum(sizes)
+    err = np.zeros_like(ref)
+    o = 0
+    for _ in range(tt2):
+        for n in sizes:
+            r = ref[o:o + n]
+            sc = np.max(np.abs(r))
+            if sc > 0:
+                err[o:o + n] = np.abs(got[o:o + n] - r) / sc
+            o += n
+    return err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form,tol", [(L.FORM_FP64_FAST, 1e-10), (L.FORM_FP32C, 1e-5)])
+def test_fast_forms_within_tolerance(form, tol, oracle):
+    """FMA FP64 and FP32-compensated forms against the oracle; error of each element
+    relative to the largest |T| of its (tt, pair) k3 row (T oscillates in k3)."""
+    cfg = L.Config(**CASES["TEB_m012"])
+    ref_s = L.Scenario(cfg)
+    ref = run_oracle(ref_s, oracle)
+    lib = L.load()
+    s = L.Scenario(cfg)
+    assert lib.los_set_default_form(form) == 0
+    try:
+        s.transfer2_init()
+    finally:
+        lib.los_set_default_form(L.FORM_FP64)
+    got = s.gather()
+    err = _row_errors(got, ref, s)
+    print(f"form {form}: max {err.max():.3e} rms {np.sqrt(np.mean(err**2)):.3e}")
+    assert err.max() < tol
+    s.close()
+    ref_s.close()

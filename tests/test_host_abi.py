@@ -1,0 +1,143 @@
+"""C-ABI surface and host-side behaviour of libsong_los.so, on the CPU (no kernel runs).
+
+* every function declared in include/*.h is exported by the library;
+* without a GPU, transfer2_init fails loudly with the HIP message (no CPU fallback);
+* SONG's early exits and the checks that fire before any GPU work
+  (source/transfer2.c:99-110, :151-157, :1408-1411, :1764-1771);
+* transfers written by the oracle in SONG's file format (one raw FP64 file per
+  transfer type, source/transfer2.c:2929-2963) load through the library's
+  transfer2_load_transfers_from_disk (:705-763).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from song_amd import lib as L
+from tests.conftest import ROOT, run_oracle
+
+HEADERS = ["include/transfer2.h", "include/song_los.h"]
+
+
+def _declared_functions():
+    names = []
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:int|void|long|double|double \*|void \*)\s*\**\s*(\w+)\s*\(",
+                             text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_all_declared_symbols_exported():
+    lib = L.load()
+    names = _declared_functions()
+    assert "transfer2_init" in names and "los_engine_run" in names and len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def _hip_devices():
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return 0
+    n = C.c_int(0)
+    if hip.hipGetDeviceCount(C.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+SMALL = dict(has_T=1, has_E=1, m_list=[0, 2], l_max_los_t=2, l_max_los_p=2, k_size=3, k3_size=6,
+             tau_size=32, l_size=4, l_max=60, dk_fine=2e-3, xx_max=1000.0)
+
+
+def test_transfer2_init_fails_loudly_without_gpu():
+    if _hip_devices() > 0:
+        pytest.skip("a GPU is present")
+    s = L.Scenario(L.Config(**SMALL))
+    with pytest.raises(L.LosError) as exc:
+        s.transfer2_init()
+    assert "los_engine_create" in str(exc.value)
+    s.close()
+
+
+def test_skip_when_no_cls_requested():
+    s = L.Scenario(L.Config(**SMALL))
+    s.set("has_cmb_bispectra", 0)
+    s.set("has_cmb_spectra", 0)
+    s.transfer2_init()  # returns _SUCCESS_ before touching the GPU
+    assert L.load().los_scenario_n_transfers(s.ptr) == 0  # has_cls is false
+    s._inited = False
+    s.close()
+
+
+@pytest.mark.parametrize("key,value,msg", [
+    ("xx_max", 10.0, "xx_max is not large enough"),
+    ("q_linstep_song", 0.0, "infinite loop"),
+])
+def test_errors_before_gpu_work(key, value, msg):
+    s = L.Scenario(L.Config(**SMALL))
+    s.set(key, value)
+    with pytest.raises(L.LosError) as exc:
+        s.transfer2_init()
+    assert msg in str(exc.value)
+    s._inited = False
+    s.close()
+
+
+def test_too_few_coarse_k3_for_cubic():
+    cfg = L.Config(**dict(SMALL, k3_size=3))
+    s = L.Scenario(cfg)
+    with pytest.raises(L.LosError) as exc:
+        s.transfer2_init()
+    assert "cannot do interpolation in k3" in str(exc.value)
+    s._inited = False
+    s.close()
+
+
+def test_load_oracle_written_transfers(oracle, tmp_path):
+    cfg = L.Config(**SMALL)
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    # oracle writes SONG's transfers_%03d.dat files
+    w = L.Scenario(cfg)
+    w.set("store_transfers_to_disk", 1)
+    w.set_string("transfers_dir", str(tmp_path))
+    assert oracle.los_ref_transfer2_init(*w.structs()) == 0, w.error_message()
+    w._inited = True
+    w.free_transfers(lambda a, b, c: oracle.los_ref_transfer2_free(a, b, c))
+    w.close()
+    files = sorted(os.listdir(tmp_path))
+    assert files[0] == "transfers_000.dat"
+    # the library reads them back
+    lib = L.load()
+    r = L.Scenario(cfg)
+    r.set("load_transfers_from_disk", 1)
+    r.set_string("transfers_dir", str(tmp_path))
+    r.transfer2_init()  # indices only, returns before the integral
+    tt2 = len(files)
+    for tt in range(tt2):
+        assert lib.transfer2_load_transfers_from_disk(r.s["ppt2"], r.s["ptr2"], tt) == 0, \
+            r.error_message()
+    got = r.gather()
+    assert np.array_equal(got, ref)
+    for tt in range(tt2):
+        lib.transfer2_free_type_level(r.s["ppt2"], r.s["ptr2"], tt)
+    r.free_transfers()
+    r.close()
+
+
+@pytest.mark.gpu
+def test_engine_rejects_unsupported_options():
+    """Options the GPU engine does not implement fail with a message, never silently."""
+    for key, val, msg in (("tau_sampling", L.TAU_CUSTOM, "tau_sampling"),):
+        s = L.Scenario(L.Config(**SMALL))
+        s.set(key, val)
+        with pytest.raises(L.LosError) as exc:
+            s.transfer2_init()
+        assert msg in str(exc.value)
+        s._inited = False
+        s.close()
