@@ -1,16 +1,25 @@
 """Benchmark of the LOS transfer stage on one MI355X.
 
-Workload c2 (the plain FP64 GPU drop-in): T+E(+B) with m = 0,1,2 and
-sources L = 0..6; 32x32 log (k1,k2) grid in [1e-5, 0.1]/Mpc (528 pairs k2 <= k1), 60
-linear coarse k3 per pair, fine k3 step 1.5e-4/Mpc; 600 tau log-spaced in [200, 14000] Mpc,
-tau0 = 14200 Mpc; 96 l in [2, 2000]; J tables on x in [0, 15000] step 0.3. Seeded synthetic
-sources (A ~ N(0,1) x (k3/0.01)^-1.5 x [Gaussian at 280 Mpc, sigma 20 Mpc + 10% late ramp]).
+Workloads (seeded synthetic sources: A ~ N(0,1) per (field,L,m,k1,k2) x (k3/0.01)^-1.5 x
+[Gaussian at 280 Mpc, sigma 20 Mpc + 10% late-time ramp]; 600 tau log-spaced in
+[200, 14000] Mpc, tau0 = 14200 Mpc; J tables on x step 0.3):
 
-A step is one pass of the LOS kernels over every pair of that workload, with the source and
-projection tables already resident in HBM (k3 spline coefficients + LOS integral). The
-JSON line reports elements/s (one element = one ptr2->transfer[tt][k1][k2][k3] value), the
-HBM roofline of the dominant kernel (algorithmic bytes / HIP-event duration) and the CPU
-oracle timed with OpenMP on a bounded sample of pairs of the same workload.
+  c2  (default) plain FP64 GPU drop-in: T+E(+B), m = 0,1,2, sources L = 0..6; 32x32 log
+      (k1,k2) grid in [1e-5, 0.1]/Mpc (528 pairs k2 <= k1), 60 linear coarse k3 per pair,
+      fine k3 step 1.5e-4/Mpc; 96 l in [2, 2000]; x in [0, 15000].
+  c3  batched: 2048 (k1,k2) pairs out of 64 k drawn log-uniformly in [1e-5, 0.2]/Mpc,
+      8 coarse k3 per pair, fine k3 step 3.5e-4/Mpc, so the triangle-limited fine k3 counts
+      run from 8 to about 400 per pair; T+E(+B), m = 0,1,2, L = 0..6, 96 l in [2, 2000];
+      every pair in one batch, the run captured in one hipGraph, tile shapes autotuned.
+  c4  c2 extended to l_max = 3000 with 128 l values (precision study of the FP32 form).
+
+A step is one pass of the LOS stage over every pair of the workload (source re-layout,
+k3 spline, LOS integral) with the source and projection tables already resident in HBM.
+The JSON line reports elements/s (one element = one ptr2->transfer[tt][k1][k2][k3] value)
+of the --form (default: the plain FP64 form, bit-identical to the FP64 CPU code), the other
+forms timed on the same resident inputs with their errors against it, the HBM roofline of
+the dominant kernel (algorithmic bytes / HIP-event duration) and the CPU oracle timed with
+OpenMP on a bounded sample of pairs of the same workload.
 """
 from __future__ import annotations
 
@@ -31,31 +40,45 @@ from song_amd import lib as L  # noqa: E402
 METRIC = ("LOS transfer elements/s (k1,k2,k3,l,m), FP32-comp vs FP64 at 1e-5 rel; p99 lat")
 HBM_PEAK_GBS = 8000.0
 
+_C2 = dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=6, l_max_los_p=6, k_size=32,
+           k_min=1e-5, k_max=0.1, k3_size=60, tau_size=600, tau_min=200.0, tau_max=14000.0,
+           tau0=14200.0, l_size=96, l_max=2000, dk_fine=1.5e-4, xx_max=15000.0, xx_step=0.3)
 WORKLOADS = {
-    "c2": dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=6, l_max_los_p=6, k_size=32,
-               k_min=1e-5, k_max=0.1, k3_size=60, tau_size=600, tau_min=200.0, tau_max=14000.0,
-               tau0=14200.0, l_size=96, l_max=2000, dk_fine=1.5e-4, xx_max=15000.0,
-               xx_step=0.3),
+    "c2": _C2,
+    "c3": dict(_C2, k_size=64, k_log=2, k_max=0.2, k3_size=8, dk_fine=3.5e-4, xx_max=6000.0),
+    "c4": dict(_C2, l_size=128, l_max=3000),
     # small workload for quick checks
     "tiny": dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=4, l_max_los_p=4, k_size=6,
                  k3_size=16, tau_size=128, l_size=12, l_max=400, dk_fine=1e-3, xx_max=1500.0),
 }
+PAIRS = {"c3": 2048}  # workloads that run an explicit list of pairs
+GRAPH = {"c3"}        # workloads whose steps replay a captured graph
 
 DESCR = {
     "c2": ("plain FP64 LOS, T+E+B m=0,1,2 L=0..6, 32x32 log (k1,k2) in [1e-5,0.1], 60 coarse "
            "k3/pair, dk=1.5e-4, 600 tau, 96 l in [2,2000]"),
+    "c3": ("batched: 2048 pairs of 64 log-uniform k in [1e-5,0.2], fine k3 8..400 per pair, "
+           "T+E+B m=0,1,2 L=0..6, 600 tau, 96 l in [2,2000], one captured graph per step"),
+    "c4": "c2 with 128 l in [2,3000]",
     "tiny": "small check workload",
 }
 
 FORMS = {"fp64": L.FORM_FP64, "fp64fast": L.FORM_FP64_FAST, "fp32c": L.FORM_FP32C}
+KERNELS = {"fp64": "los_integrate_exact", "fp64fast": "<StreamFP64> (stream or sweep, tuned)",
+           "fp32c": "<StreamFP32C> (stream or sweep, tuned)"}
 
 
-def _err():
-    return C.create_string_buffer(L.ERRSZ)
+def select_pairs(cfg: L.Config, n: int):
+    """n distinct (k1, k2) pairs of the grid, a seeded random choice, in grid order."""
+    allp = [(i1, i2) for i1 in range(cfg.k_size) for i2 in range(i1 + 1)]
+    if n >= len(allp):
+        return allp
+    pick = np.sort(np.random.default_rng(cfg.seed).choice(len(allp), n, replace=False))
+    return [allp[i] for i in pick]
 
 
-def cpu_baseline(cfg: L.Config, target_s: float):
-    """Time the CPU oracle on a bounded sample of pairs of the same workload."""
+def cpu_baseline(cfg: L.Config, pairs, target_s: float):
+    """Time the CPU oracle on a bounded sample of the workload's pairs."""
     opath = os.path.join(ROOT, "oracle", "liblos_oracle.so")
     o = C.CDLL(opath)
     o.los_ref_run_pairs.argtypes = [C.c_void_p] * 8 + [C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -65,8 +88,6 @@ def cpu_baseline(cfg: L.Config, target_s: float):
     o.los_ref_transfer2_free.argtypes = [C.c_void_p] * 3
     scen = L.Scenario(cfg)
     s = scen.s
-    nk = cfg.k_size
-    pairs = [(i1, i2) for i1 in range(nk) for i2 in range(i1 + 1)]
     # spread the sample over the whole pair list; grow it until target_s is used
     total_el, total_t, done, stride = 0, 0.0, 0, 24
     order = pairs[::stride] + [p for q, p in enumerate(pairs) if q % stride]
@@ -96,19 +117,41 @@ def cpu_baseline(cfg: L.Config, target_s: float):
                       f"({total_el} elements, {total_t:.1f} s, OpenMP static over k3 per pair)"}
 
 
-def row_normalised_errors(got, ref, sizes, tt2_size):
-    """|got - ref| / max|ref| over the k3 row of the same (tt, pair) (the transfer
-    functions oscillate in k3, so single zero crossings are not used as scales)."""
-    err = np.zeros_like(ref)
+def relative_errors(got, ref, sizes, tt2_size):
+    """Two normalisations of |got - ref| for each element of a [tt][pair][k3] array.
+
+    field: divided by max|ref| over the whole (l,m) transfer function of the same tt
+           (all pairs and k3), i.e. the sup-norm relative error of T_lm;
+    row:   divided by max|ref| over the k3 row of the same (tt, pair); stricter, since
+           rows whose values cancel to far below the field's scale are judged on
+           their own scale.
+    Single elements are never used as scales: the transfers oscillate in k3."""
+    F = int(sum(sizes))
+    g = np.asarray(got).reshape(tt2_size, F)
+    r = np.asarray(ref).reshape(tt2_size, F)
+    d = np.abs(g - r)
+    fscale = np.abs(r).max(axis=1, keepdims=True)
+    field = np.divide(d, fscale, out=np.zeros_like(d), where=fscale > 0)
+    row = np.zeros_like(d)
     o = 0
-    for _tt in range(tt2_size):
-        for n in sizes:
-            r = ref[o:o + n]
-            scale = np.max(np.abs(r))
-            if scale > 0:
-                err[o:o + n] = np.abs(got[o:o + n] - r) / scale
-            o += n
-    return err
+    for n in sizes:
+        sc = np.abs(r[:, o:o + n]).max(axis=1, keepdims=True)
+        row[:, o:o + n] = np.divide(d[:, o:o + n], sc, out=np.zeros_like(d[:, o:o + n]),
+                                    where=sc > 0)
+        o += n
+    return field, row
+
+
+def loaded_columns(scen: L.Scenario, pairs):
+    """Indices, inside one transfer type of gather(), of the k3 values of `pairs`, and the
+    k3 count of each of those pairs."""
+    offs = scen.pair_offsets()
+    cols, sizes = [], []
+    for p in sorted(pairs):
+        o, n = offs[p]
+        cols.append(np.arange(o, o + n))
+        sizes.append(n)
+    return np.concatenate(cols), sizes
 
 
 def main(argv=None):
@@ -122,6 +165,8 @@ def main(argv=None):
     ap.add_argument("--also", default="fp32c,fp64fast",
                     help="other forms timed on the same resident inputs and checked "
                          "against the fp64 form ('' for none)")
+    ap.add_argument("--graph", choices=["auto", "on", "off"], default="auto",
+                    help="replay each step as one captured hipGraph (auto: batched workloads)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--errors-out", default="", help="write per-(l,m) errors as JSON here")
@@ -131,81 +176,92 @@ def main(argv=None):
               "supported (the (k1,k2) pairs would shard across GPUs with no communication)",
               file=sys.stderr)
         return 2
+    graph = args.graph == "on" or (args.graph == "auto" and args.workload in GRAPH)
 
-    lib = L.load()
     cfg = L.Config(**WORKLOADS[args.workload])
     t0 = time.time()
     scen = L.Scenario(cfg)
+    scen.prepare()
     print(f"# inputs built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
-    s = scen.s
-    err = _err()
-    if lib.los_transfer2_prepare(*scen.structs()) != L.SUCCESS:
-        raise L.LosError("los_transfer2_prepare: " + scen.error_message())
-    scen._inited = True
-    eng = C.c_void_p()
-    L.check(lib.los_engine_create(0, C.byref(eng), err), err, "engine_create")
+    all_pairs = [(i1, i2) for i1 in range(cfg.k_size) for i2 in range(i1 + 1)]
+    pairs = select_pairs(cfg, PAIRS[args.workload]) if args.workload in PAIRS else all_pairs
+    eng = L.Engine(0)
     t0 = time.time()
-    L.check(lib.los_engine_load(eng, s["ppr"], s["ppr2"], s["ppt2"], s["pbs"], s["pbs2"],
-                                s["ptr2"], 0, cfg.k_size, err), err, "engine_load")
+    if args.workload in PAIRS:
+        eng.load_pairs(scen, pairs)
+    else:
+        eng.load(scen)
     print(f"# tables uploaded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
-    n_el = lib.los_engine_n_elements(eng)
-    alg_bytes = lib.los_engine_alg_bytes(eng)
+    n_el = eng.n_elements()
+    cols, sizes = loaded_columns(scen, pairs)
 
     def run_form(form):
-        L.check(lib.los_engine_set_form(eng, FORMS[form], err), err, "set_form")
-        ms = C.c_float()
-        for _ in range(args.warmup):
-            L.check(lib.los_engine_run(eng, C.byref(ms), err), err, "engine_run")
-        los_ms, spl_ms = [], []
+        eng.set_form(FORMS[form])
+        if form != "fp64":
+            eng.autotune(2)
+        kern = []
+        for _ in range(max(1, args.warmup)):
+            eng.run()
+            kern.append(eng.last_kernel_ms())
+        los_ms = min(k[0] for k in kern)
+        prep_ms = min(k[1] for k in kern)
+        if graph:
+            eng.capture()
+            eng.run()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            L.check(lib.los_engine_run(eng, C.byref(ms), err), err, "engine_run")
-            a, b = C.c_float(), C.c_float()
-            lib.los_engine_last_kernel_ms(eng, C.byref(a), C.byref(b))
-            los_ms.append(a.value)
-            spl_ms.append(b.value)
+            eng.run()
         wall = time.perf_counter() - t0
-        L.check(lib.los_engine_store(eng, s["ptr2"], err), err, "engine_store")
-        out = scen.gather()
+        if not graph:
+            los_ms, prep_ms = eng.last_kernel_ms()
+        eng.store(scen)
+        tt_out = scen.gather()
+        tt2 = tt_out.size // int(scen.k3_sizes()[:, 0].sum())
+        out = tt_out.reshape(tt2, -1)[:, cols].ravel()
         ms_step = 1e3 * wall / args.steps
-        los_avg = sum(los_ms) / len(los_ms)
-        print(f"# {form}: {ms_step:.2f} ms/step, LOS kernel {los_avg:.2f} ms", file=sys.stderr,
-              flush=True)
-        return {"ms_per_step": ms_step, "value": n_el / (ms_step * 1e-3), "kernel_ms": los_avg,
-                "prep_ms": sum(spl_ms) / len(spl_ms)}, out
+        print(f"# {form}: {ms_step:.2f} ms/step, LOS kernel {los_ms:.2f} ms, prep {prep_ms:.2f} ms",
+              file=sys.stderr, flush=True)
+        res = {"ms_per_step": ms_step, "value": n_el / (ms_step * 1e-3), "kernel_ms": los_ms,
+               "prep_ms": prep_ms, "graph": graph, "alg_bytes": eng.alg_bytes()}
+        if los_ms > 0:
+            res["hbm_gbs"] = res["alg_bytes"] / (los_ms * 1e-3) / 1e9
+        if form != "fp64":
+            kind, nk, ch, pers = eng.get_shape(int(form == "fp32c"))
+            res["tile_shape"] = ({"kernel": "sweep", "nk": nk} if kind == 1 else
+                                 {"kernel": "stream", "nk": nk, "ch": ch, "persistent": pers})
+        return res, out, tt2
 
     forms = [args.form] + [f for f in args.also.split(",") if f and f != args.form]
     if "fp64" not in forms:
         forms.append("fp64")
     results, outs = {}, {}
+    tt2 = 0
     for f in forms:
-        results[f], outs[f] = run_form(f)
-    lib.los_engine_destroy(eng)
+        results[f], outs[f], tt2 = run_form(f)
+    eng.close()
 
-    sizes = [int(r[0]) for r in scen.k3_sizes()]
-    tt2 = len(outs["fp64"]) // sum(sizes)
-    lm_count = tt2 // (2 * cfg.has_E * (1 if max(cfg.m_list) > 0 else 0) + cfg.has_E + cfg.has_T)
     per_lm = {}
     for f in forms:
         if f == "fp64":
             continue
-        e = row_normalised_errors(outs[f], outs["fp64"], sizes, tt2)
-        results[f]["max_rel_err_vs_fp64"] = float(e.max())
-        results[f]["rms_rel_err_vs_fp64"] = float(np.sqrt(np.mean(e ** 2)))
-        per_tt = e.reshape(tt2, -1)
-        worst = int(np.argmax(per_tt.max(axis=1)))
-        results[f]["worst_tt"] = worst
-        per_lm[f] = {"max": per_tt.max(axis=1).tolist(),
-                     "rms": np.sqrt(np.mean(per_tt ** 2, axis=1)).tolist()}
+        field, row = relative_errors(outs[f], outs["fp64"], sizes, tt2)
+        results[f]["speedup_vs_fp64"] = results[f]["value"] / results["fp64"]["value"]
+        results[f]["max_rel_err_vs_fp64"] = float(field.max())
+        results[f]["rms_rel_err_vs_fp64"] = float(np.sqrt(np.mean(field ** 2)))
+        results[f]["max_row_rel_err_vs_fp64"] = float(row.max())
+        results[f]["rms_row_rel_err_vs_fp64"] = float(np.sqrt(np.mean(row ** 2)))
+        results[f]["worst_tt"] = int(np.argmax(field.max(axis=1)))
+        per_lm[f] = {"max": field.max(axis=1).tolist(),
+                     "rms": np.sqrt(np.mean(field ** 2, axis=1)).tolist(),
+                     "row_max": row.max(axis=1).tolist()}
     scen.close()
     if args.errors_out:
         with open(args.errors_out, "w") as fh:
-            json.dump({"per_tt": per_lm, "tt2_size": tt2, "n_lm": lm_count}, fh)
+            json.dump({"workload": args.workload, "per_tt": per_lm, "tt2_size": tt2}, fh)
 
     head = results[args.form]
-    achieved = alg_bytes / (head["kernel_ms"] * 1e-3) / 1e9
-    kernel_name = {"fp64": "los_integrate_exact", "fp64fast": "los_integrate_stream<StreamFP64>",
-                   "fp32c": "los_integrate_stream<StreamFP32C>"}[args.form]
+    alg_bytes = head["alg_bytes"]
+    achieved = head.get("hbm_gbs", 0.0)
     line = {
         "metric": METRIC, "value": head["value"], "unit": "elements/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -213,15 +269,15 @@ def main(argv=None):
         "vs_baseline": None, "dtype": "f64" if args.form != "fp32c" else "f32-compensated",
         "data": "synthetic (seeded source model, FP64 J tables built on the host)",
         "config": {"workload": f"{args.workload}: {DESCR[args.workload]}", "form": args.form,
-                   "elements": n_el, "pairs": cfg.k_size * (cfg.k_size + 1) // 2},
+                   "elements": n_el, "pairs": len(pairs)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel_name,
-                     "kernel_ms": head["kernel_ms"], "prep_ms": head["prep_ms"],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": KERNELS[args.form], "kernel_ms": head["kernel_ms"],
                      "alg_bytes": alg_bytes},
         "forms": results,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(cfg, pairs, args.cpu_seconds)
     print(json.dumps(line), flush=True)
     return 0
 

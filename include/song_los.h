@@ -60,22 +60,51 @@ int los_engine_load(los_engine *eng, struct precision *ppr, struct precision2 *p
                     struct perturbs2 *ppt2, struct bessels *pbs, struct bessels2 *pbs2,
                     struct transfers2 *ptr2, int k1_begin, int k1_end, char *errmsg);
 
-/* Run the LOS kernels on the loaded pairs; *ms gets the GPU time (HIP events). */
+/* The same for an explicit list of (index_k1, index_k2) pairs (0 <= index_k2 <=
+   index_k1 < ppt2->k_size), in any order and with any k3 counts: the batched
+   form. Results are stored per pair as by los_engine_load. */
+int los_engine_load_pairs(los_engine *eng, struct precision *ppr, struct precision2 *ppr2,
+                          struct perturbs2 *ppt2, struct bessels *pbs, struct bessels2 *pbs2,
+                          struct transfers2 *ptr2, const int *index_k1, const int *index_k2,
+                          int n_pairs, char *errmsg);
+
+/* Run the LOS kernels on the loaded pairs; *ms gets the GPU time (HIP events).
+   Runs the captured graph when los_engine_capture was called for this form. */
 int los_engine_run(los_engine *eng, float *ms, char *errmsg);
 
-/* GPU time of the last run's main LOS kernel alone (ms) and its launch count. */
+/* GPU time of the last eager run's LOS kernel and of its source preparation
+   (re-layout + k3 spline), in ms; -1 after a graph run (only the total is known). */
 int los_engine_last_kernel_ms(los_engine *eng, float *ms_los, float *ms_spline);
+
+/* Kernel organisation and tile shape of the FMA-FP64 (fp32 = 0) or
+   FP32-compensated (fp32 = 1) form. kind 0, the "stream" kernel: nk fine k3
+   per workgroup, ch time steps per LDS chunk, persistent = 1 for the chip-sized
+   grid fed by the device work queue. kind 1, the "sweep" kernel: one wave per
+   (nk fine k3, m, 64 l) over per-(k3, tau) tables made before it (ch and
+   persistent unused). los_engine_autotune times every shape that fits on the
+   loaded batch (reps timed launches each) and keeps the fastest; the choice is
+   cached per batch shape in the process. Runs tune on first use unless a shape
+   was set after the load. */
+int los_engine_autotune(los_engine *eng, int reps, char *errmsg);
+int los_engine_set_shape(los_engine *eng, int fp32, int kind, int nk, int ch, int persistent,
+                         char *errmsg);
+void los_engine_get_shape(los_engine *eng, int fp32, int *kind, int *nk, int *ch, int *persistent);
+
+/* Capture the whole run of the loaded batch (source re-layout, k3 spline, LOS
+   kernel with its queue reset) in one hipGraph; later los_engine_run calls of
+   the same form replay it. A new load drops the graph. */
+int los_engine_capture(los_engine *eng, char *errmsg);
 
 /* Copy the results of the loaded pairs into ptr2->transfer[tt][k1][k2][k]
    (levels must be allocated) and update ptr2->count_memorised_transfers. */
 int los_engine_store(los_engine *eng, struct transfers2 *ptr2, char *errmsg);
 
-/* Element count (tt x fine k3 summed over loaded pairs) and live-work count
-   (sum over elements of live (tau,L) terms, the reference's skip applied). */
+/* Element count (tt x fine k3 summed over loaded pairs). */
 long los_engine_n_elements(los_engine *eng);
-double los_engine_live_terms(los_engine *eng);
-/* Algorithmic bytes of one run: sources read once + J window read once +
-   outputs written once. */
+/* Algorithmic bytes of one LOS kernel launch of the current form: the coarse
+   source columns once (twice with the cubic k3 spline's second derivatives),
+   the live part of the J table once (FP32 for the FP32-compensated form), the
+   outputs once. */
 double los_engine_alg_bytes(los_engine *eng);
 /* The HIP stream the engine launches on (hipStream_t as void*). */
 void *los_engine_stream(los_engine *eng);
@@ -105,7 +134,8 @@ typedef struct los_config {
   int m_size;
   int l_max_los_t;    /* sources L = 0..l_max_los_t for T */
   int l_max_los_p;    /* and for E, B */
-  /* (k1,k2) grid: k_size values, log (k_log=1) or linear spaced in [k_min,k_max] */
+  /* (k1,k2) grid: k_size values in [k_min,k_max], log (k_log=1) or linear (0) spaced,
+     or (k_log=2) drawn log-uniformly from the seed and sorted */
   int k_size;
   double k_min, k_max;
   int k_log;

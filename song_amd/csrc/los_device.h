@@ -20,6 +20,28 @@ typedef struct {
   int k1, k2;    /* indices in ppt2->k */
 } los_pair;
 
+/* A job: one block of the output array, [tt][ostride], with n_lm (l,m) lanes
+   per field (the first n_lm (l,m) offsets of the projection table). */
+typedef struct {
+  long obase;          /* offset of the block in out */
+  int ostride;         /* columns of the block (fine k3 values of the job) */
+  int n_lm;            /* (l,m) lanes */
+  int ttT, ttE, ttB;   /* first row of each field inside the block */
+} los_job;
+
+/* A tile: npos consecutive entries of the work order, all of one job. */
+typedef struct {
+  int pos0, npos;
+} los_tile;
+
+/* Kernel organisation and tile shape of the fast forms:
+   kind 0: "stream" kernel, nk fine k3 per workgroup, ch time steps per LDS
+           chunk, persistent = 1 for the chip-sized grid fed by a work queue;
+   kind 1: "sweep" kernel, one wave per (tile of nk fine k3, m, 64 l). */
+typedef struct {
+  int kind, nk, ch, persistent;
+} los_shape;
+
 /* Everything the LOS kernels read; pointers are device pointers. */
 typedef struct {
   /* pairs and fine k3 */
@@ -32,12 +54,24 @@ typedef struct {
   const double *src;    /* sources, per pair [tp][tau][kc] */
   const double *dd;     /* spline second derivatives, same layout */
   int F;                /* fine k3 total */
-  /* Work order: the fine k3 sorted by value (all pairs together), dealt out
-     so that each XCD of the chip receives a contiguous run of that order:
-     blocks that run together on one XCD then read neighbouring J rows and
-     share its L2. forder[s] is the fine index at sorted position s. */
+  /* Work order: n_pos positions, forder[s] the fine index at position s. The
+     positions of one job are contiguous and sorted by k3, so that workgroups
+     that run together on one XCD read neighbouring J rows and share its L2.
+     Position s belongs to job pos_job[s] and writes column ocol[s] of it. */
   const int *forder;
-  int f_per_xcd;        /* sorted positions per XCD (ceil(F/8)) */
+  int n_pos;
+  const int *pos_job;
+  const int *ocol;
+  const los_job *jobs;
+  int n_lm_max;         /* largest n_lm of the jobs */
+  const los_tile *tiles;  /* tiles of the tiled kernels (NK positions or fewer) */
+  /* XCD x owns positions [xcd_pos[x], xcd_pos[x+1]) (one fine k3 per
+     workgroup) or tiles [xcd_tile[x], xcd_tile[x+1]); the ranges are
+     contiguous and balanced by estimated cost, and each XCD takes its range
+     from the end down (the costliest, largest k3, first within a job). */
+  int xcd_pos[9];
+  int xcd_tile[9];
+  int *queue;           /* 8 tile counters of the persistent kernels */
   int tp2_size;
   int cubic_k3;         /* 1: cubic spline in k3, 0: linear */
 
@@ -55,7 +89,7 @@ typedef struct {
   double xx_step;
   int X;
   int L_size;
-  int n_lm;
+  int n_lm;             /* (l,m) offsets of the table (its row stride) */
   const int *l_of;      /* l value per lm */
   const int *m_of;      /* m value per lm */
   const int *mi_of;     /* index of m in the m-list per lm */
@@ -64,13 +98,26 @@ typedef struct {
   int has_T, has_E, has_B;
   int LmaxT, LmaxP;
   int jTT, jEE, jEB;    /* J type indices (-1 if absent) */
-  int ttT, ttE, ttB;    /* first transfer type of each field */
   /* source type of (field, L, m-index), -1 if absent */
   int srcT[LOS_MAX_L][LOS_MAX_MI];
   int srcE[LOS_MAX_L][LOS_MAX_MI];
   int srcB[LOS_MAX_L][LOS_MAX_MI];
 
-  /* output [tt][F] */
+  /* Sweep kernels (FMA-FP64 and FP32-compensated forms). One wave handles one
+     m and 64 consecutive l: lane slot lp = mi * n_lp + il (n_lp = l count
+     rounded up to 64). */
+  const void *Jp;        /* J re-laid out [jt][L][x][lp] (FP64 or FP32) */
+  const int *xminp;      /* [jt][L][lp] first live x index, INT_MAX for empty slots */
+  const int *lm_of_lp;   /* (l,m) offset of each slot, -1 for empty slots */
+  int n_lp;              /* l slots per m */
+  int m_size;            /* entries of the m-list */
+  int m_of_mi[LOS_MAX_MI]; /* m value of each m-list entry */
+  int n_wave;            /* waves per tile: m_size * n_lp / 64 */
+  void *ixa;             /* per (f, t): index_x and a_J ({int, float} or {int, pad, double}) */
+  void *sf;              /* per (f, t, mi, L): S x trapezoid weight of T, E, B (+ pad) */
+  const int *tau_end_fw; /* per (f, wave): time steps before every row of the wave is dead */
+
+  /* output blocks of the jobs */
   double *out;
 } los_params;
 

@@ -14,80 +14,51 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "los_device.h"
 #include "los_internal.h"
 
-extern "C" hipError_t los_launch_transpose(const los_pair *pairs, int npairs, int nc_max,
-                                           const double *raw, double *out, int ntau, int tp2,
-                                           hipStream_t s);
-extern "C" hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double *kc,
-                                            const double *spl, const double *src, double *dd,
-                                            int ntau, int tp2, hipStream_t s);
-extern "C" hipError_t los_launch_integrate_exact(const los_params *P, int nthreads, hipStream_t s);
-extern "C" hipError_t los_launch_stream_fp64(const los_params *P, hipStream_t s);
-extern "C" hipError_t los_launch_stream_fp32c(const los_params *P, hipStream_t s);
-extern "C" hipError_t los_launch_round_j32(const double *src, float *dst, long n, hipStream_t s);
+#include "los_engine_impl.h"
 
-struct los_engine {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int form = LOS_FORM_FP64;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  float ms_spline = 0, ms_los = 0;
-
-  // projection tables (resident across loads)
-  const void *J_owner = nullptr;
-  double *dJ = nullptr;
-  float *dJ32 = nullptr;
-  size_t nJ = 0;
-  int *dxmin = nullptr, *dminx = nullptr;
-  double *dxx = nullptr;
-  int *dl_of = nullptr, *dm_of = nullptr, *dmi_of = nullptr;
-  int X = 0, L_size = 0, J_size = 0, n_lm = 0;
-  double J_window_bytes = 0;
-
-  // time grid
-  double *dt0mt = nullptr, *ddtau = nullptr;
-  int ntau = 0;
-
-  // loaded pairs
-  std::vector<los_pair> pairs;
-  los_pair *dpairs = nullptr;
-  double *dkc = nullptr, *dkf = nullptr, *dspl = nullptr;
-  int *dfpair = nullptr, *dfsrc = nullptr, *dfbr = nullptr, *dforder = nullptr;
-  double *draw = nullptr, *dsrc = nullptr, *ddd = nullptr, *dout = nullptr;
-  long src_total = 0;
-  int nc_max = 0;
-  int F = 0;
-  int tt2_size = 0;
-  los_params P;
-  bool loaded = false;
-
-  double *staging = nullptr;  // pinned
-  size_t staging_bytes = 0;
-};
-
-#define HIPCHK(expr, err)                                                                  \
-  do {                                                                                     \
-    hipError_t e_ = (expr);                                                                \
-    if (e_ != hipSuccess) {                                                                \
-      snprintf((err), _ERRORMSGSIZE_, "%s: %s failed: %s", __func__, #expr,                \
-               hipGetErrorString(e_));                                                     \
-      return _FAILURE_;                                                                    \
-    }                                                                                      \
-  } while (0)
-
-template <class T>
-static void dfree(T *&p)
+// Copy host memory to the device through the pinned staging buffer.
+int upload(los_engine *e, void *dst, const void *src, size_t bytes, char *err)
 {
-  if (p) (void)hipFree((void *)p);
-  p = nullptr;
+  const char *s = (const char *)src;
+  char *d = (char *)dst;
+  while (bytes > 0) {
+    size_t n = std::min(bytes, e->staging_bytes);
+    memcpy(e->staging, s, n);
+    HIPCHK(hipMemcpyAsync(d, e->staging, n, hipMemcpyHostToDevice, e->stream), err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+    s += n;
+    d += n;
+    bytes -= n;
+  }
+  return _SUCCESS_;
+}
+
+static void drop_graph(los_engine *e)
+{
+  if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->graph_exec = nullptr;
+  e->graph = nullptr;
+  e->graph_form = -1;
 }
 
 static void free_batch(los_engine *e)
 {
+  drop_graph(e);
+  dfree(e->dtau_end_fw);
+  if (e->dixa) (void)hipFree(e->dixa);
+  if (e->dsf) (void)hipFree(e->dsf);
+  e->dixa = e->dsf = nullptr;
+  e->sweep_fp32 = -1;
+  e->sweep_bytes = 0;
   dfree(e->dpairs);
   dfree(e->dkc);
   dfree(e->dkf);
@@ -95,6 +66,13 @@ static void free_batch(los_engine *e)
   dfree(e->dfsrc);
   dfree(e->dfbr);
   dfree(e->dforder);
+  dfree(e->dposjob);
+  dfree(e->docol);
+  dfree(e->djobs);
+  for (int i = 0; i < 4; ++i) {
+    dfree(e->dtiles[i]);
+    e->htiles[i].clear();
+  }
   dfree(e->dspl);
   dfree(e->draw);
   dfree(e->dsrc);
@@ -115,6 +93,10 @@ static void free_tables(los_engine *e)
   dfree(e->dl_of);
   dfree(e->dm_of);
   dfree(e->dmi_of);
+  dfree(e->dJp);
+  dfree(e->dJp32);
+  dfree(e->dxminp);
+  dfree(e->dlm_of_lp);
   e->J_owner = nullptr;
 }
 
@@ -133,6 +115,8 @@ extern "C" int los_engine_create(int device, los_engine **out, char *err)
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&e->ev[i]), err);
   e->staging_bytes = 64ul << 20;
   HIPCHK(hipHostMalloc((void **)&e->staging, e->staging_bytes, hipHostMallocDefault), err);
+  HIPCHK(hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, device), err);
+  HIPCHK(hipMalloc((void **)&e->dqueue, 64 * sizeof(int)), err);
   *out = e;
   return _SUCCESS_;
 }
@@ -144,6 +128,7 @@ extern "C" void los_engine_destroy(los_engine *e)
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   free_batch(e);
   free_tables(e);
+  dfree(e->dqueue);
   if (e->staging) (void)hipHostFree(e->staging);
   for (int i = 0; i < 4; ++i)
     if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
@@ -182,33 +167,6 @@ extern "C" double los_engine_batch_bytes(struct precision2 *ppr2, struct perturb
       b += 8.0 * (ptr2->tt2_size + 4.0) * ptr2->k_size_k1k2[i1][i2];
     }
   return b;
-}
-
-// Copy host memory to the device through the pinned staging buffer.
-static int upload(los_engine *e, void *dst, const void *src, size_t bytes, char *err)
-{
-  const char *s = (const char *)src;
-  char *d = (char *)dst;
-  while (bytes > 0) {
-    size_t n = std::min(bytes, e->staging_bytes);
-    memcpy(e->staging, s, n);
-    HIPCHK(hipMemcpyAsync(d, e->staging, n, hipMemcpyHostToDevice, e->stream), err);
-    HIPCHK(hipStreamSynchronize(e->stream), err);
-    s += n;
-    d += n;
-    bytes -= n;
-  }
-  return _SUCCESS_;
-}
-
-template <class T>
-static int upload_vec(los_engine *e, T *&dptr, const std::vector<T> &v, char *err)
-{
-  dfree(dptr);
-  size_t bytes = std::max<size_t>(1, v.size()) * sizeof(T);
-  HIPCHK(hipMalloc((void **)&dptr, bytes), err);
-  if (!v.empty()) return upload(e, dptr, v.data(), v.size() * sizeof(T), err);
-  return _SUCCESS_;
 }
 
 // Build the device projection table [jt][L][x][lm] for x < X.
@@ -275,7 +233,38 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
       }
       hminx[lm] = mn;
     }
+  // sweep layout: slot lp = mi * n_lp + il for every (l, m-list entry)
+  const int m_size = ppr2->m_size, n_l = ptr2->l_size;
+  const int n_lp = (n_l + 63) / 64 * 64, NLP = m_size * n_lp;
+  std::vector<int> lm_of_lp(NLP, -1);
+  for (int il = 0; il < n_l; ++il)
+    for (int im = 0; im < m_size; ++im)
+      if (im <= ppr2->index_m_max[ptr2->l[il]]) lm_of_lp[im * n_lp + il] = ptr2->lm_array[il][im];
+  const size_t nJp = (size_t)J_size * L_size * X * NLP;
+  std::vector<double> hJp(nJp, 0.0);
+  std::vector<int> hxminp((size_t)J_size * L_size * NLP, INT_MAX);
+  for (size_t r = 0; r < (size_t)J_size * L_size; ++r)
+    for (int lp = 0; lp < NLP; ++lp) {
+      const int lm = lm_of_lp[lp];
+      if (lm < 0) continue;
+      hxminp[r * NLP + lp] = hxmin[r * n_lm + lm];
+      for (int x = 0; x < X; ++x) hJp[(r * X + x) * NLP + lp] = hJ[(r * X + x) * n_lm + lm];
+    }
+  e->wave_minx.assign(NLP / 64, INT_MAX);
+  for (int lp = 0; lp < NLP; ++lp)
+    if (lm_of_lp[lp] >= 0) e->wave_minx[lp / 64] = std::min(e->wave_minx[lp / 64], hminx[lm_of_lp[lp]]);
+
   free_tables(e);
+  e->n_lp = n_lp;
+  e->m_size = m_size;
+  e->n_wave = NLP / 64;
+  e->nJp = nJp;
+  HIPCHK(hipMalloc((void **)&e->dJp, nJp * sizeof(double)), err);
+  if (upload(e, e->dJp, hJp.data(), nJp * sizeof(double), err)) return _FAILURE_;
+  if (upload_vec(e, e->dxminp, hxminp, err) || upload_vec(e, e->dlm_of_lp, lm_of_lp, err))
+    return _FAILURE_;
+  e->minx_all = INT_MAX;
+  for (int v : hminx) e->minx_all = std::min(e->minx_all, v);
   HIPCHK(hipMalloc((void **)&e->dJ, nJ * sizeof(double)), err);
   if (upload(e, e->dJ, hJ.data(), nJ * sizeof(double), err)) return _FAILURE_;
   if (upload_vec(e, e->dxmin, hxmin, err)) return _FAILURE_;
@@ -295,10 +284,43 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
   return _SUCCESS_;
 }
 
-extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct precision2 *ppr2,
-                               struct perturbs2 *ppt2, struct bessels *pbs,
-                               struct bessels2 *pbs2, struct transfers2 *ptr2, int k1_begin,
-                               int k1_end, char *err)
+// Tiles dealt to the 8 XCDs as contiguous ranges of about equal estimated
+// cost (the cost of a tile is that of its last, largest-k3, position).
+void split_xcd(const std::vector<los_tile> &tiles, const std::vector<double> &pos_cost,
+                      int out[9])
+{
+  const int nt = (int)tiles.size();
+  std::vector<double> pre(nt + 1, 0.0);
+  for (int t = 0; t < nt; ++t)
+    pre[t + 1] = pre[t] + pos_cost[tiles[t].pos0 + tiles[t].npos - 1];
+  out[0] = 0;
+  int t = 0;
+  for (int x = 1; x < 8; ++x) {
+    const double goal = pre[nt] * x / 8.0;
+    while (t < nt && pre[t + 1] <= goal) ++t;
+    out[x] = t;
+  }
+  out[8] = nt;
+}
+
+int nk_slot(int nk) { return nk == 1 ? 0 : nk == 2 ? 1 : nk == 4 ? 2 : 3; }
+
+// Tiles of at most nk consecutive positions that never straddle two jobs
+// (job_end: one past the last position of each job, ascending).
+std::vector<los_tile> make_tiles(const std::vector<int> &job_end, int nk)
+{
+  std::vector<los_tile> t;
+  int p = 0;
+  for (int end : job_end)
+    for (; p < end; p += nk) t.push_back(los_tile{p, std::min(nk, end - p)});
+  return t;
+}
+
+extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
+                                     struct precision2 *ppr2, struct perturbs2 *ppt2,
+                                     struct bessels *pbs, struct bessels2 *pbs2,
+                                     struct transfers2 *ptr2, const int *pk1, const int *pk2,
+                                     int npairs, char *err)
 {
   HIPCHK(hipSetDevice(e->device), err);
   if (ptr2->tau_sampling != sources_tau_sampling) {
@@ -321,7 +343,14 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
              std::max(LmaxT, LmaxP), ppr2->m_size);
     return _FAILURE_;
   }
+  for (int q = 0; q < npairs; ++q)
+    if (pk1[q] < 0 || pk1[q] >= ppt2->k_size || pk2[q] < 0 || pk2[q] > pk1[q]) {
+      snprintf(err, _ERRORMSGSIZE_, "los_engine_load_pairs: pair %d = (%d,%d) is not a (k1,k2) "
+               "pair with 0 <= index_k2 <= index_k1 < %d", q, pk1[q], pk2[q], ppt2->k_size);
+      return _FAILURE_;
+    }
   free_batch(e);
+  e->shape_set[0] = e->shape_set[1] = false;  // shapes are chosen per batch
   const int ntau = ppt2->tau_size;
   const int tp2 = ppt2->tp2_size;
 
@@ -339,8 +368,9 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
   std::vector<int> fpair, fsrc, fbr;
   long src_off = 0;
   std::vector<double> klist(std::max(1, ptr2->k3_size_max));
-  for (int i1 = k1_begin; i1 < k1_end; ++i1)
-    for (int i2 = 0; i2 <= i1; ++i2) {
+  for (int q = 0; q < npairs; ++q) {
+    {
+      const int i1 = pk1[q], i2 = pk2[q];
       if (!ppt2->has_allocated_sources[i1]) {
         snprintf(err, _ERRORMSGSIZE_, "los_engine_load: sources of index_k1=%d are not allocated", i1);
         return _FAILURE_;
@@ -404,6 +434,7 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
       src_off += (long)tp2 * ntau * p.nc;
       e->pairs.push_back(p);
     }
+  }
   e->F = (int)kf.size();
   // processing order: fine k3 sorted by value over all pairs (ties by index)
   std::vector<int> forder(e->F);
@@ -420,6 +451,60 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
       for (int i2 = 0; i2 <= i1; ++i2) kmax_all = std::max(kmax_all, ptr2->k_max_k1k2[i1][i2]);
     if (build_tables(e, ppr2, ppt2, pbs2, ptr2, kmax_all, err) == _FAILURE_) return _FAILURE_;
   }
+
+  // estimated cost of each sorted position: the number of time steps before
+  // x = k (tau0 - tau) falls below every row's first live index, plus a fixed
+  // per-workgroup part
+  e->pos_cost.assign(e->F, 0.0);
+  for (int sp = 0; sp < e->F; ++sp) {
+    const double k = kf[forder[sp]];
+    int lo = 0, hi = ntau;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int)(k * t0mt[mid] / pbs2->xx_step) >= e->minx_all) lo = mid + 1; else hi = mid;
+    }
+    e->pos_cost[sp] = lo + 16.0;
+  }
+  // sweep kernels: live time steps of each (fine k3, wave class), and the cost
+  // of a position as the sum over its waves
+  {
+    const int W = e->n_wave;
+    auto live = [&](double k, int mx) {
+      int lo = 0, hi = ntau;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)(k * t0mt[mid] / pbs2->xx_step) >= mx) lo = mid + 1; else hi = mid;
+      }
+      return lo;
+    };
+    std::vector<int> tend((size_t)e->F * W);
+    for (int f = 0; f < e->F; ++f)
+      for (int w = 0; w < W; ++w) tend[(size_t)f * W + w] = live(kf[f], e->wave_minx[w]);
+    e->pos_cost_sweep.assign(e->F, 0.0);
+    for (int sp = 0; sp < e->F; ++sp) {
+      double c = 0;
+      for (int w = 0; w < W; ++w) c += tend[(size_t)forder[sp] * W + w] + 4.0;
+      e->pos_cost_sweep[sp] = c;
+    }
+    if (upload_vec(e, e->dtau_end_fw, tend, err)) return _FAILURE_;
+  }
+
+  // one job: the whole batch, output [tt][F] at the fine index
+  std::vector<int> posjob(e->F, 0);
+  std::vector<los_job> jobs(1);
+  jobs[0].obase = 0;
+  jobs[0].ostride = e->F;
+  jobs[0].n_lm = e->n_lm;
+  jobs[0].ttT = ptr2->index_tt2_T;
+  jobs[0].ttE = ptr2->index_tt2_E;
+  jobs[0].ttB = ptr2->index_tt2_B;
+  for (int i = 0; i < 4; ++i) {
+    e->htiles[i] = make_tiles(std::vector<int>(1, e->F), 1 << i);
+    if (upload_vec(e, e->dtiles[i], e->htiles[i], err)) return _FAILURE_;
+  }
+  if (upload_vec(e, e->dposjob, posjob, err) || upload_vec(e, e->docol, forder, err) ||
+      upload_vec(e, e->djobs, jobs, err))
+    return _FAILURE_;
 
   if (upload_vec(e, e->dt0mt, t0mt, err) || upload_vec(e, e->ddtau, dtau, err) ||
       upload_vec(e, e->dpairs, e->pairs, err) || upload_vec(e, e->dkc, kc, err) ||
@@ -455,7 +540,22 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
   P.fsrc = e->dfsrc;
   P.fbr = e->dfbr;
   P.forder = e->dforder;
-  P.f_per_xcd = (e->F + 7) / 8;
+  P.n_pos = e->F;
+  P.pos_job = e->dposjob;
+  P.ocol = e->docol;
+  P.jobs = e->djobs;
+  P.n_lm_max = e->n_lm;
+  P.tiles = e->dtiles[0];
+  split_xcd(e->htiles[0], e->pos_cost, P.xcd_pos);
+  P.Jp = e->dJp;
+  P.xminp = e->dxminp;
+  P.lm_of_lp = e->dlm_of_lp;
+  P.n_lp = e->n_lp;
+  P.n_wave = e->n_wave;
+  P.m_size = e->m_size;
+  for (int im = 0; im < LOS_MAX_MI; ++im) P.m_of_mi[im] = im < ppr2->m_size ? ppr2->m[im] : -1;
+  P.tau_end_fw = e->dtau_end_fw;
+  P.queue = e->dqueue;
   P.src = e->dsrc;
   P.dd = e->ddd;
   P.F = e->F;
@@ -483,9 +583,6 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
   P.jTT = pbs2->has_J_TT ? pbs2->index_J_TT : -1;
   P.jEE = pbs2->has_J_EE ? pbs2->index_J_EE : -1;
   P.jEB = pbs2->has_J_EB ? pbs2->index_J_EB : -1;
-  P.ttT = ptr2->index_tt2_T;
-  P.ttE = ptr2->index_tt2_E;
-  P.ttB = ptr2->index_tt2_B;
   for (int L = 0; L < LOS_MAX_L; ++L)
     for (int im = 0; im < LOS_MAX_MI; ++im) {
       P.srcT[L][im] = P.srcE[L][im] = P.srcB[L][im] = -1;
@@ -502,7 +599,239 @@ extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct prec
     return _FAILURE_;
   }
   P.out = e->dout;
+  HIPCHK(los_stream_set_attrs(tp2), err);
   e->loaded = true;
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct precision2 *ppr2,
+                               struct perturbs2 *ppt2, struct bessels *pbs,
+                               struct bessels2 *pbs2, struct transfers2 *ptr2, int k1_begin,
+                               int k1_end, char *err)
+{
+  std::vector<int> k1, k2;
+  for (int i1 = k1_begin; i1 < k1_end; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2) {
+      k1.push_back(i1);
+      k2.push_back(i2);
+    }
+  return los_engine_load_pairs(e, ppr, ppr2, ppt2, pbs, pbs2, ptr2, k1.data(), k2.data(),
+                               (int)k1.size(), err);
+}
+
+// Re-layout and k3 spline of the loaded sources (enqueued, not waited for).
+int enqueue_prep(los_engine *e, char *err)
+{
+  const int tp2 = e->P.tp2_size;
+  const int np = (int)e->pairs.size();
+  HIPCHK(los_launch_transpose(e->dpairs, np, e->nc_max, e->draw, e->dsrc, e->ntau, tp2, e->stream),
+         err);
+  if (e->P.cubic_k3)
+    HIPCHK(los_launch_spline_tau(e->dpairs, np, e->dkc, e->dspl, e->dsrc, e->ddd, e->ntau, tp2,
+                                 e->stream),
+           err);
+  return _SUCCESS_;
+}
+
+// The LOS kernel of the current form with the current tile shape (enqueued).
+// (For a sweep shape, its per-(k3, tau) tables first; `mid`, if given, is
+// recorded between the two so that the sweep kernel can be timed alone.)
+static int enqueue_los(los_engine *e, const los_shape *sh, char *err, hipEvent_t mid = nullptr)
+{
+  if (e->form == LOS_FORM_FP64) {
+    if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
+    HIPCHK(los_launch_integrate_exact(&e->P, 0, e->stream), err);
+    return _SUCCESS_;
+  }
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  los_params P = e->P;
+  P.tiles = e->dtiles[nk_slot(sh->nk)];
+  if (sh->kind == 1) {
+    if (e->sweep_fp32 != fp32) {
+      snprintf(err, _ERRORMSGSIZE_, "enqueue_los: sweep tables not prepared for this form");
+      return _FAILURE_;
+    }
+    P.Jp = fp32 ? (const void *)e->dJp32 : (const void *)e->dJp;
+    HIPCHK(los_launch_sweep_prep(fp32, &P, e->stream), err);
+    if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
+    split_xcd(e->htiles[nk_slot(sh->nk)], e->pos_cost_sweep, P.xcd_tile);
+    HIPCHK(los_launch_sweep(fp32, &P, sh->nk, e->stream), err);
+    return _SUCCESS_;
+  }
+  if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
+  split_xcd(e->htiles[nk_slot(sh->nk)], e->pos_cost, P.xcd_tile);
+  HIPCHK(los_launch_stream(fp32, &P, sh, e->n_cu, e->stream), err);
+  return _SUCCESS_;
+}
+
+// Device tables a sweep shape needs (allocated outside any capture).
+int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
+{
+  if (sh == nullptr || sh->kind != 1 || e->form == LOS_FORM_FP64) return _SUCCESS_;
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  if (fp32 && e->dJp32 == nullptr) {
+    HIPCHK(hipMalloc((void **)&e->dJp32, std::max<size_t>(1, e->nJp) * sizeof(float)), err);
+    HIPCHK(los_launch_round_j32(e->dJp, e->dJp32, (long)e->nJp, e->stream), err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+  }
+  if (e->sweep_fp32 == fp32) return _SUCCESS_;
+  if (e->dixa) (void)hipFree(e->dixa);
+  if (e->dsf) (void)hipFree(e->dsf);
+  e->dixa = e->dsf = nullptr;
+  e->sweep_fp32 = -1;
+  const int NL = std::max(e->P.LmaxT, e->P.LmaxP) + 1;
+  const size_t n = (size_t)e->F * e->ntau;
+  const size_t rb = fp32 ? 4 : 8;
+  const size_t ixa_b = std::max<size_t>(1, n) * (fp32 ? 8 : 16);
+  const size_t sf_b = std::max<size_t>(1, n * e->m_size * NL) * 4 * rb;
+  HIPCHK(hipMalloc(&e->dixa, ixa_b), err);
+  HIPCHK(hipMalloc(&e->dsf, sf_b), err);
+  e->P.ixa = e->dixa;
+  e->P.sf = e->dsf;
+  e->sweep_fp32 = fp32;
+  e->sweep_bytes = ixa_b + sf_b;
+  return _SUCCESS_;
+}
+
+int ensure_j32(los_engine *e, char *err)
+{
+  if (e->form == LOS_FORM_FP32C && e->dJ32 == nullptr) {
+    // FP32 copy of the resident J table, made once per table upload
+    HIPCHK(hipMalloc((void **)&e->dJ32, std::max<size_t>(1, e->nJ) * sizeof(float)), err);
+    HIPCHK(los_launch_round_j32(e->dJ, e->dJ32, (long)e->nJ, e->stream), err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+  }
+  e->P.J32 = e->dJ32;
+  return _SUCCESS_;
+}
+
+// Autotuner cache: best shape per (form, (l,m) count, source types, sources L,
+// time steps, fine-k3 count rounded to a power of two).
+typedef std::tuple<int, int, int, int, int, int> tune_key;
+static std::map<tune_key, los_shape> g_tuned;
+static std::mutex g_tuned_mu;
+
+static tune_key key_of(const los_engine *e)
+{
+  int fb = 0;
+  while ((1 << fb) < e->F) ++fb;
+  return tune_key(e->form, e->n_lm, e->P.tp2_size, std::max(e->P.LmaxT, e->P.LmaxP), e->ntau, fb);
+}
+
+extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
+{
+  if (!e->loaded) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: nothing loaded");
+    return _FAILURE_;
+  }
+  if (e->form == LOS_FORM_FP64) return _SUCCESS_;  // one kernel shape only
+  HIPCHK(hipSetDevice(e->device), err);
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  const tune_key key = key_of(e);
+  {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end() && reps >= 0) {
+      e->shape[fp32] = it->second;
+      e->shape_set[fp32] = true;
+      return _SUCCESS_;
+    }
+  }
+  if (reps < 1) reps = 1;
+  if (ensure_j32(e, err) || enqueue_prep(e, err)) return _FAILURE_;
+  los_shape cand[32];
+  const int nc = los_shapes(fp32, e->P.tp2_size, cand, 32);
+  if (nc == 0) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: no tile shape fits %d source types in LDS",
+             e->P.tp2_size);
+    return _FAILURE_;
+  }
+  float best = 1e30f;
+  int ibest = 0;
+  for (int i = 0; i < nc; ++i) {
+    if (ensure_sweep(e, &cand[i], err)) return _FAILURE_;
+    if (enqueue_los(e, &cand[i], err)) return _FAILURE_;  // warm-up
+    HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
+    for (int r = 0; r < reps; ++r)
+      if (enqueue_los(e, &cand[i], err)) return _FAILURE_;
+    HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
+    HIPCHK(hipEventSynchronize(e->ev[2]), err);
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev[1], e->ev[2]), err);
+    if (ms < best) {
+      best = ms;
+      ibest = i;
+    }
+  }
+  e->shape[fp32] = cand[ibest];
+  e->shape_set[fp32] = true;
+  drop_graph(e);
+  std::lock_guard<std::mutex> g(g_tuned_mu);
+  g_tuned[key] = cand[ibest];
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_set_shape(los_engine *e, int fp32, int kind, int nk, int ch,
+                                    int persistent, char *err)
+{
+  los_shape cand[32];
+  const int nc = los_shapes(fp32 != 0, e->loaded ? e->P.tp2_size : 1, cand, 32);
+  for (int i = 0; i < nc; ++i)
+    if (cand[i].kind == kind && cand[i].nk == nk && (kind == 1 || (cand[i].ch == ch &&
+                                                     cand[i].persistent == (persistent != 0)))) {
+      e->shape[fp32 != 0] = cand[i];
+      e->shape_set[fp32 != 0] = true;
+      drop_graph(e);
+      return _SUCCESS_;
+    }
+  snprintf(err, _ERRORMSGSIZE_, "los_engine_set_shape: no tile shape kind=%d nk=%d ch=%d "
+           "persistent=%d for this batch", kind, nk, ch, persistent);
+  return _FAILURE_;
+}
+
+extern "C" void los_engine_get_shape(los_engine *e, int fp32, int *kind, int *nk, int *ch,
+                                     int *persistent)
+{
+  const los_shape &s = e->shape[fp32 != 0];
+  if (kind) *kind = s.kind;
+  if (nk) *nk = s.nk;
+  if (ch) *ch = s.ch;
+  if (persistent) *persistent = s.persistent;
+}
+
+// Shape of the current form, tuned on first use.
+int current_shape(los_engine *e, const los_shape **sh, char *err)
+{
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  if (e->form != LOS_FORM_FP64 && !e->shape_set[fp32])
+    if (los_engine_autotune(e, 2, err)) return _FAILURE_;
+  *sh = &e->shape[fp32];
+  return ensure_sweep(e, *sh, err);
+}
+
+extern "C" int los_engine_capture(los_engine *e, char *err)
+{
+  if (!e->loaded) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_capture: nothing loaded");
+    return _FAILURE_;
+  }
+  HIPCHK(hipSetDevice(e->device), err);
+  const los_shape *sh;
+  if (ensure_j32(e, err) || current_shape(e, &sh, err)) return _FAILURE_;
+  drop_graph(e);
+  HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), err);
+  int rc = enqueue_prep(e, err);
+  if (rc == _SUCCESS_) rc = enqueue_los(e, sh, err);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+  if (rc != _SUCCESS_) {
+    if (g) (void)hipGraphDestroy(g);
+    return _FAILURE_;
+  }
+  HIPCHK(ec, err);
+  e->graph = g;
+  HIPCHK(hipGraphInstantiate(&e->graph_exec, e->graph, nullptr, nullptr, 0), err);
+  e->graph_form = e->form;
   return _SUCCESS_;
 }
 
@@ -513,35 +842,25 @@ extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
     return _FAILURE_;
   }
   HIPCHK(hipSetDevice(e->device), err);
-  const int tp2 = e->P.tp2_size;
-  const int np = (int)e->pairs.size();
+  const los_shape *sh;
+  if (ensure_j32(e, err) || current_shape(e, &sh, err)) return _FAILURE_;
+  if (e->graph_exec && e->graph_form == e->form) {
+    // captured run: one graph launch; only the total time is known
+    HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
+    HIPCHK(hipGraphLaunch(e->graph_exec, e->stream), err);
+    HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
+    HIPCHK(hipEventSynchronize(e->ev[2]), err);
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e->ev[0], e->ev[2]), err);
+    e->ms_spline = -1;
+    e->ms_los = -1;
+    if (ms) *ms = t;
+    return _SUCCESS_;
+  }
   HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
-  HIPCHK(los_launch_transpose(e->dpairs, np, e->nc_max, e->draw, e->dsrc, e->ntau, tp2, e->stream),
-         err);
-  if (e->P.cubic_k3)
-    HIPCHK(los_launch_spline_tau(e->dpairs, np, e->dkc, e->dspl, e->dsrc, e->ddd, e->ntau, tp2,
-                                 e->stream),
-           err);
+  if (enqueue_prep(e, err)) return _FAILURE_;
   HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
-  if (e->form == LOS_FORM_FP32C && e->dJ32 == nullptr) {
-    // FP32 copy of the resident J table, made once per table upload
-    HIPCHK(hipMalloc((void **)&e->dJ32, std::max<size_t>(1, e->nJ) * sizeof(float)), err);
-    HIPCHK(los_launch_round_j32(e->dJ, e->dJ32, (long)e->nJ, e->stream), err);
-    HIPCHK(hipStreamSynchronize(e->stream), err);
-    HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
-  }
-  e->P.J32 = e->dJ32;
-  switch (e->form) {
-    case LOS_FORM_FP64:
-      HIPCHK(los_launch_integrate_exact(&e->P, 0, e->stream), err);
-      break;
-    case LOS_FORM_FP64_FAST:
-      HIPCHK(los_launch_stream_fp64(&e->P, e->stream), err);
-      break;
-    default:
-      HIPCHK(los_launch_stream_fp32c(&e->P, e->stream), err);
-      break;
-  }
+  if (enqueue_los(e, sh, err)) return _FAILURE_;
   HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
   HIPCHK(hipEventSynchronize(e->ev[2]), err);
   HIPCHK(hipEventElapsedTime(&e->ms_spline, e->ev[0], e->ev[1]), err);
@@ -583,13 +902,12 @@ extern "C" int los_engine_store(los_engine *e, struct transfers2 *ptr2, char *er
 
 extern "C" long los_engine_n_elements(los_engine *e) { return (long)e->tt2_size * e->F; }
 
-extern "C" double los_engine_live_terms(los_engine *e)
-{
-  (void)e;
-  return -1.0;
-}
-
 extern "C" double los_engine_alg_bytes(los_engine *e)
 {
-  return 8.0 * e->src_total + e->J_window_bytes + 8.0 * (double)e->tt2_size * e->F;
+  // each coarse source column once (and its spline second derivatives when the
+  // k3 interpolation is cubic), the live part of the J table once at the
+  // precision the form reads it in, each output once
+  const double src = 8.0 * e->src_total * (e->P.cubic_k3 ? 2 : 1);
+  const double jb = e->J_window_bytes * (e->form == LOS_FORM_FP32C ? 0.5 : 1.0);
+  return src + jb + 8.0 * (double)e->tt2_size * e->F;
 }

@@ -127,17 +127,21 @@ template <int NL>
 __global__ void __launch_bounds__(LOS_EXACT_BLOCK_MAX)
 los_integrate_exact(los_params P)
 {
-  extern __shared__ double smem[];
+  HIP_DYNAMIC_SHARED(double, smem)
   double *sS = smem;                                  // [tp2][CH]
   double *sA = sS + (long)P.tp2_size * CH;            // a_J[CH]
   int *sIX = (int *)(sA + CH);                        // index_x[CH]
   __shared__ int s_tau_end;
 
-  // workgroups are dispatched round-robin over the 8 XCDs: give XCD (b % 8)
-  // the sorted positions [(b % 8) * f_per_xcd, ...) in order
-  const int spos = (blockIdx.x & 7) * P.f_per_xcd + (blockIdx.x >> 3);
-  if (spos >= P.F) return;
+  // workgroups are dispatched round-robin over the 8 XCDs: XCD (b % 8) gets
+  // its contiguous range of sorted positions, costliest (largest k3) first
+  const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
+  if (q >= P.xcd_pos[xcd + 1] - P.xcd_pos[xcd]) return;
+  const int spos = P.xcd_pos[xcd + 1] - 1 - q;
+  const los_job job = P.jobs[P.pos_job[spos]];
+  if ((int)(blockIdx.y * blockDim.x) >= job.n_lm) return;  // no lane of this job here
   const int f = P.forder[spos];
+  const long ocell = job.obase + P.ocol[spos];
   const int pidx = P.fpair[f];
   const los_pair pr = P.pairs[pidx];
   const double k = P.kf[f];
@@ -152,7 +156,7 @@ los_integrate_exact(los_params P)
   const double cb = bw * bw * bw - bw;
 
   const int lm = blockIdx.y * blockDim.x + threadIdx.x;
-  const bool active = lm < P.n_lm;
+  const bool active = lm < job.n_lm;
 
   // Live tau range of this lane: index_x falls with tau, so the terms with
   // index_x >= min_xmin form a prefix of the time grid.
@@ -309,7 +313,7 @@ los_integrate_exact(los_params P)
     v /= 4.;
     v /= den;
     v /= 2.;
-    P.out[(long)(P.ttT + lm) * P.F + f] = v;
+    P.out[ocell + (long)(job.ttT + lm) * job.ostride] = v;
   }
   if (P.has_E) {
     double d = accEd;
@@ -323,7 +327,7 @@ los_integrate_exact(los_params P)
     v /= 4.;
     v /= den;
     v /= 2.;
-    P.out[(long)(P.ttE + lm) * P.F + f] = v;
+    P.out[ocell + (long)(job.ttE + lm) * job.ostride] = v;
   }
   if (P.has_B) {
     double d = accBd;
@@ -334,7 +338,7 @@ los_integrate_exact(los_params P)
     v /= 4.;
     v /= den;
     v /= 2.;
-    P.out[(long)(P.ttB + lm) * P.F + f] = v;
+    P.out[ocell + (long)(job.ttB + lm) * job.ostride] = v;
   }
 }
 
@@ -381,14 +385,16 @@ hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double
 hipError_t los_launch_integrate_exact(const los_params *P, int nthreads, hipStream_t s)
 {
   (void)nthreads;
-  if (P->F == 0 || P->n_lm == 0) return hipSuccess;
+  if (P->n_pos == 0 || P->n_lm_max == 0) return hipSuccess;
   // (l,m) lanes split over gridDim.y blocks of at most LOS_EXACT_BLOCK_MAX lanes
-  const int nblk = (P->n_lm + LOS_EXACT_BLOCK_MAX - 1) / LOS_EXACT_BLOCK_MAX;
-  const int per = (P->n_lm + nblk - 1) / nblk;
+  const int nblk = (P->n_lm_max + LOS_EXACT_BLOCK_MAX - 1) / LOS_EXACT_BLOCK_MAX;
+  const int per = (P->n_lm_max + nblk - 1) / nblk;
   const int threads = ((per + 63) / 64) * 64;
   const size_t shmem = (size_t)P->tp2_size * CH * sizeof(double) + CH * (sizeof(double) + sizeof(int));
   const int NLr = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  const dim3 grid(8 * P->f_per_xcd, nblk);
+  int most = 0;
+  for (int x = 0; x < 8; ++x) most = most > P->xcd_pos[x + 1] - P->xcd_pos[x] ? most : P->xcd_pos[x + 1] - P->xcd_pos[x];
+  const dim3 grid(8 * most, nblk);
   switch (NLr) {
     case 1: return launch_exact_nl<1>(P, grid, threads, shmem, s);
     case 2: return launch_exact_nl<2>(P, grid, threads, shmem, s);

@@ -1,20 +1,40 @@
 // los_kernels_fast.hip — the LOS integral in its FP64-with-FMA and
-// FP32-compensated forms (los_stream.h), plus the FP32 copy of the J table.
+// FP32-compensated forms (los_stream.h) for every tile shape the engine's
+// autotuner chooses from, plus the FP32 copy of the J table.
 #include <hip/hip_runtime.h>
 
 #include "los_stream.h"
-
-#ifndef LOS_NK_FP64
-#define LOS_NK_FP64 8
-#endif
-#ifndef LOS_NK_FP32
-#define LOS_NK_FP32 8
-#endif
+#include "los_sweep.h"
 
 __global__ void __launch_bounds__(256) los_round_j32(const double *src, float *dst, long n)
 {
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     dst[i] = (float)src[i];
+}
+
+// tile shapes: NK fine k3 per workgroup x CH tau values per LDS chunk
+#define LOS_SHAPES(X) X(2, 16) X(2, 32) X(4, 16) X(4, 32) X(8, 16) X(8, 32)
+
+template <class Pol>
+static hipError_t launch_shape(const los_params *P, const los_shape *sh, int n_cu,
+                               hipStream_t s)
+{
+#define LOS_CASE(NK_, CH_) \
+  if (sh->nk == NK_ && sh->ch == CH_) return launch_stream<Pol, NK_, CH_>(P, sh->persistent, n_cu, s);
+  LOS_SHAPES(LOS_CASE)
+#undef LOS_CASE
+  return hipErrorInvalidValue;
+}
+
+template <class Pol>
+static hipError_t set_attrs(int tp2_size)
+{
+  hipError_t e = hipSuccess;
+#define LOS_ATTR(NK_, CH_) \
+  if (e == hipSuccess) e = stream_attr_one<Pol, NK_, CH_>(tp2_size);
+  LOS_SHAPES(LOS_ATTR)
+#undef LOS_ATTR
+  return e;
 }
 
 extern "C" {
@@ -26,14 +46,49 @@ hipError_t los_launch_round_j32(const double *src, float *dst, long n, hipStream
   return hipGetLastError();
 }
 
-hipError_t los_launch_stream_fp64(const los_params *P, hipStream_t s)
+// fp32: 0 for the FP64-with-FMA form, 1 for the FP32-compensated form
+hipError_t los_launch_stream(int fp32, const los_params *P, const los_shape *sh, int n_cu,
+                             hipStream_t s)
 {
-  return launch_stream<StreamFP64, LOS_NK_FP64>(P, s);
+  return fp32 ? launch_shape<StreamFP32C>(P, sh, n_cu, s) : launch_shape<StreamFP64>(P, sh, n_cu, s);
 }
 
-hipError_t los_launch_stream_fp32c(const los_params *P, hipStream_t s)
+hipError_t los_stream_set_attrs(int tp2_size)
 {
-  return launch_stream<StreamFP32C, LOS_NK_FP32>(P, s);
+  hipError_t e = set_attrs<StreamFP64>(tp2_size);
+  return e == hipSuccess ? set_attrs<StreamFP32C>(tp2_size) : e;
+}
+
+// Sweep organisation: per-(k3, tau) tables, then the sweep kernel with tiles of
+// nk positions (1, 2 or 4).
+hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s)
+{
+  return fp32 ? launch_sweep_prep<float>(P, s) : launch_sweep_prep<double>(P, s);
+}
+
+hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s)
+{
+  switch (nk) {
+    case 1: return fp32 ? launch_sweep<StreamFP32C, 1>(P, s) : launch_sweep<StreamFP64, 1>(P, s);
+    case 2: return fp32 ? launch_sweep<StreamFP32C, 2>(P, s) : launch_sweep<StreamFP64, 2>(P, s);
+    case 4: return fp32 ? launch_sweep<StreamFP32C, 4>(P, s) : launch_sweep<StreamFP64, 4>(P, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Candidate shapes: the stream shapes (both dispatch orders) that fit the LDS of
+// one workgroup, then the sweep shapes.
+int los_shapes(int fp32, int tp2_size, los_shape *out, int cap)
+{
+  int n = 0;
+  const size_t real = fp32 ? sizeof(float) : sizeof(double);
+#define LOS_LIST(NK_, CH_)                                                        \
+  if ((size_t)NK_ * ((size_t)tp2_size * CH_ + CH_) * real + 8 * 1024 <= 160 * 1024) \
+    for (int p = 0; p < 2 && n < cap; ++p) out[n++] = los_shape{0, NK_, CH_, p};
+  LOS_SHAPES(LOS_LIST)
+#undef LOS_LIST
+  for (int nk = 1; nk <= 4 && n < cap; nk *= 2) out[n++] = los_shape{1, nk, 0, 0};
+  return n;
 }
 
 }  // extern "C"

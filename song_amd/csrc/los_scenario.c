@@ -81,6 +81,12 @@ void los_config_default(los_config *c)
   c->verbose = 0;
 }
 
+static int cmp_double(const void *a, const void *b)
+{
+  const double x = *(const double *)a, y = *(const double *)b;
+  return (x > y) - (x < y);
+}
+
 static unsigned long long splitmix64(unsigned long long x)
 {
   x += 0x9E3779B97F4A7C15ULL;
@@ -235,10 +241,20 @@ int los_scenario_create(const los_config *cfg, los_scenario **out, char *err)
   los_alloc(ppt2->k, c->k_size * sizeof(double), err);
   for (int i = 0; i < c->k_size; ++i) {
     if (c->k_size == 1) { ppt2->k[i] = c->k_min; break; }
-    if (c->k_log)
+    if (c->k_log == 2) {
+      /* log-uniform draws in [k_min, k_max] from the scenario seed (sorted below) */
+      unsigned long long u = splitmix64(splitmix64(c->seed ^ 0x6B5F6772696421ULL) + (unsigned)i);
+      double t = (double)(u >> 11) * 0x1.0p-53;
+      ppt2->k[i] = exp(log(c->k_min) + t * (log(c->k_max) - log(c->k_min)));
+    } else if (c->k_log)
       ppt2->k[i] = exp(log(c->k_min) + i * (log(c->k_max) - log(c->k_min)) / (c->k_size - 1));
     else
       ppt2->k[i] = c->k_min + i * (c->k_max - c->k_min) / (c->k_size - 1);
+  }
+  if (c->k_log == 2) {
+    qsort(ppt2->k, c->k_size, sizeof(double), cmp_double);
+    for (int i = 1; i < c->k_size; ++i)
+      los_test(ppt2->k[i] <= ppt2->k[i - 1], err, "k draws %d and %d coincide", i - 1, i);
   }
   /* coarse k3 grids (source/perturbations2.c:1965-1984, linear sampling) */
   los_alloc(ppt2->k3, c->k_size * sizeof(double **), err);

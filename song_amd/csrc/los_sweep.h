@@ -1,0 +1,280 @@
+// los_sweep.h — LOS integral for the fast forms (FP64 with FMA, FP32 with
+// compensated accumulation) on gfx950, "sweep" organisation.
+//
+// A wave (one workgroup) owns a tile of at most NK fine k3 of one job, one m
+// and 64 consecutive l (lane slot lp = mi * n_lp + il). Everything that depends
+// on (k3, tau) alone is wave-uniform and read with scalar loads from tables
+// made by los_sweep_prep: index_x and a_J (source/transfer2.c:2195-2202) and
+// the sources interpolated in k3 (source/transfer2.c:2700-2706) times the
+// trapezoid weight (source/transfer2.c:2317), for the wave's m and every L.
+// Each lane then gathers its J row values, J[jt][L][x][lp] with the slots of
+// one m contiguous so that a wave reads 64 consecutive values per x, and the
+// NK elements of the tile share a J load whenever their index_x coincide
+// (neighbouring k3 in the sorted work order). The reference's skip of rows
+// below index_xmin_J (source/transfer2.c:2232) is a select on the
+// interpolated J. No LDS and no barriers: waves are independent.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "los_device.h"
+
+template <class R>
+struct SweepIXA;
+template <>
+struct SweepIXA<float> {
+  int ix;
+  float a;
+};
+template <>
+struct SweepIXA<double> {
+  int ix, pad;
+  double a;
+};
+
+template <class R>
+struct SweepS {
+  R T, E, B, pad;
+};
+
+// ---------------------------------------------------------------------------
+// Per (fine k3, time step): index_x, a_J and S x dtau of every (m, L, field).
+// One thread per (f, t), t fastest (the tau-contiguous source layout).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(256)
+los_sweep_prep(los_params P, int NL)
+{
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long)P.F * P.ntau) return;
+  const int f = (int)(id / P.ntau);
+  const int t = (int)(id - (long)f * P.ntau);
+  const double k = P.kf[f];
+  const double x = k * P.t0mt[t];
+  const int ix = (int)(x / P.xx_step);
+  SweepIXA<R> o;
+  o.ix = ix;
+  o.a = (R)((P.xx[ix + 1] - x) / P.xx_step);
+  ((SweepIXA<R> *)P.ixa)[id] = o;
+
+  const los_pair pr = P.pairs[P.fpair[f]];
+  const int br = P.fbr[f];
+  const double *kc = P.kc + pr.kc_off;
+  const double ks = P.kf[P.fsrc[f]];
+  const double h = kc[br + 1] - kc[br];
+  const double bw = (ks - kc[br]) / h;
+  const double aw = 1 - bw;
+  const double ca = aw * aw * aw - aw, cb = bw * bw * bw - bw;
+  const double dt = P.dtau[t];
+  auto src = [&](int tp) -> double {
+    if (tp < 0) return 0.0;
+    const long base = pr.src_off + ((long)tp * pr.nc + br) * P.ntau + t;
+    const double *S = P.src + base;
+    double v = aw * S[0] + bw * S[P.ntau];
+    if (P.cubic_k3) {
+      const double *D = P.dd + base;
+      v += (ca * D[0] + cb * D[P.ntau]) * h * h / 6.0;
+    }
+    return v * dt;
+  };
+  SweepS<R> *out = (SweepS<R> *)P.sf + (id * P.m_size) * NL;
+  for (int mi = 0; mi < P.m_size; ++mi)
+    for (int L = 0; L < NL; ++L) {
+      SweepS<R> s;
+      s.T = (R)(P.has_T && L <= P.LmaxT ? src(P.srcT[L][mi]) : 0.0);
+      s.E = (R)(P.has_E && L <= P.LmaxP ? src(P.srcE[L][mi]) : 0.0);
+      s.B = (R)(P.has_B && L <= P.LmaxP ? src(P.srcB[L][mi]) : 0.0);
+      s.pad = 0;
+      out[mi * NL + L] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep kernel. Block = one wave; blockIdx.y = wave class (mi, l chunk);
+// blockIdx.x = XCD-aware tile index (tiles of XCD b % 8, costliest first).
+// ---------------------------------------------------------------------------
+template <class Pol, int NK, int NL>
+__global__ void __launch_bounds__(64)
+los_sweep(los_params P)
+{
+  typedef typename Pol::real R;
+  typedef typename Pol::part_t Q;
+  typedef typename Pol::acc_t A;
+  constexpr int FOLD = 32;  // time steps per partial sum
+
+  const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
+  if (q >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return;
+  const int tile = P.xcd_tile[xcd + 1] - 1 - q;
+  const int w = blockIdx.y;
+  const int nchunk = P.n_lp >> 6;
+  const int mi = w / nchunk;
+  const int m = P.m_of_mi[mi];
+  const int lp = mi * P.n_lp + (w - mi * nchunk) * 64 + threadIdx.x;
+  const los_tile tl = P.tiles[tile];
+  const los_job job = P.jobs[P.pos_job[tl.pos0]];
+  const int lm = P.lm_of_lp[lp];
+  const bool active = lm >= 0 && lm < job.n_lm;
+  if (__ballot(active) == 0) return;
+
+  const int NLP = P.m_size * P.n_lp;
+  int f[NK];
+  int tend = 0;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    f[j] = P.forder[tl.pos0 + min(j, tl.npos - 1)];
+    tend = max(tend, P.tau_end_fw[(long)f[j] * P.n_wave + w]);
+  }
+
+  // first live x index of each row the lane reads (INT_MAX: never live)
+  int xmT[NL], xmE[NL], xmM[NL];
+#pragma unroll
+  for (int L = 0; L < NL; ++L) {
+    xmT[L] = xmE[L] = xmM[L] = INT_MAX;
+    if (!active || m > L || L >= P.L_size) continue;
+    if (P.has_T && L <= P.LmaxT && P.jTT >= 0)
+      xmT[L] = P.xminp[((long)P.jTT * P.L_size + L) * NLP + lp];
+    if ((P.has_E || P.has_B) && L <= P.LmaxP) {
+      if (P.jEE >= 0) xmE[L] = P.xminp[((long)P.jEE * P.L_size + L) * NLP + lp];
+      if (P.jEB >= 0) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
+    }
+  }
+  const R *Jb = (const R *)P.Jp + lp;
+  const long rowstep = (long)P.X * NLP;
+  const SweepIXA<R> *__restrict__ ixa = (const SweepIXA<R> *)P.ixa;
+  const SweepS<R> *__restrict__ sf = (const SweepS<R> *)P.sf;
+  const bool doT = P.has_T, doP = P.has_E || P.has_B, mix = P.jEB >= 0;
+  const R emix = m != 0 ? (R)1 : (R)0;  // E takes the B->E mixing only for m != 0
+
+  A aT[NK], aE[NK], aB[NK];
+#pragma unroll
+  for (int j = 0; j < NK; ++j) aT[j] = aE[j] = aB[j] = A{};
+
+  for (int t0 = 0; t0 < tend; t0 += FOLD) {
+    const int t1 = min(tend, t0 + FOLD);
+    Q pT[NK], pE[NK], pB[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) pT[j] = pE[j] = pB[j] = 0;
+    for (int t = t0; t < t1; ++t) {
+      int ix[NK];
+      R a[NK];
+#pragma unroll
+      for (int j = 0; j < NK; ++j) {
+        const SweepIXA<R> v = ixa[(long)f[j] * P.ntau + t];
+        ix[j] = v.ix;
+        a[j] = v.a;
+      }
+      // all J loads of this time step first (elements whose index_x equals the
+      // previous element's reuse its values), then the arithmetic
+      R t0v[NK][NL], t1v[NK][NL], e0v[NK][NL], e1v[NK][NL], q0v[NK][NL], q1v[NK][NL];
+#pragma unroll
+      for (int j = 0; j < NK; ++j) {
+        if (j > 0 && ix[j] == ix[j - 1]) {
+#pragma unroll
+          for (int L = 0; L < NL; ++L) {
+            t0v[j][L] = t0v[j - 1][L];
+            t1v[j][L] = t1v[j - 1][L];
+            e0v[j][L] = e0v[j - 1][L];
+            e1v[j][L] = e1v[j - 1][L];
+            q0v[j][L] = q0v[j - 1][L];
+            q1v[j][L] = q1v[j - 1][L];
+          }
+          continue;
+        }
+        const long o0 = (long)ix[j] * NLP, o1 = o0 + NLP;
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          t0v[j][L] = t1v[j][L] = e0v[j][L] = e1v[j][L] = q0v[j][L] = q1v[j][L] = 0;
+          if (m > L) continue;
+          if (doT && L <= P.LmaxT) {
+            const R *row = Jb + (long)(P.jTT * P.L_size + L) * rowstep;
+            t0v[j][L] = row[o0];
+            t1v[j][L] = row[o1];
+          }
+          if (doP && L <= P.LmaxP) {
+            const R *re = Jb + (long)(P.jEE * P.L_size + L) * rowstep;
+            e0v[j][L] = re[o0];
+            e1v[j][L] = re[o1];
+            if (mix) {
+              const R *rm = Jb + (long)(P.jEB * P.L_size + L) * rowstep;
+              q0v[j][L] = rm[o0];
+              q1v[j][L] = rm[o1];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NK; ++j) {
+        const SweepS<R> *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * NL;
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          if (m > L) continue;
+          const SweepS<R> s = sp[L];
+          if (doT && L <= P.LmaxT) {
+            R Jv = t1v[j][L] + a[j] * (t0v[j][L] - t1v[j][L]);
+            Jv = ix[j] >= xmT[L] ? Jv : (R)0;
+            pT[j] += Jv * s.T;
+          }
+          if (doP && L <= P.LmaxP) {
+            R Je = e1v[j][L] + a[j] * (e0v[j][L] - e1v[j][L]);
+            R Jm = q1v[j][L] + a[j] * (q0v[j][L] - q1v[j][L]);
+            Je = ix[j] >= xmE[L] ? Je : (R)0;
+            Jm = ix[j] >= xmM[L] ? Jm : (R)0;
+            pE[j] += Je * s.E + Jm * (emix * s.B);
+            pB[j] += Je * s.B - Jm * s.E;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+      Pol::fold(aT[j], pT[j]);
+      Pol::fold(aE[j], pE[j]);
+      Pol::fold(aB[j], pB[j]);
+    }
+  }
+  if (!active) return;
+  const double den = 2. * P.l_of[lm] + 1;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    if (j >= tl.npos) continue;
+    const long o = job.obase + P.ocol[tl.pos0 + j];
+    if (P.has_T) P.out[o + (long)(job.ttT + lm) * job.ostride] = Pol::value(aT[j]) * 0.5 / 4. / den / 2.;
+    if (P.has_E) P.out[o + (long)(job.ttE + lm) * job.ostride] = Pol::value(aE[j]) * 0.5 / 4. / den / 2.;
+    if (P.has_B) P.out[o + (long)(job.ttB + lm) * job.ostride] = Pol::value(aB[j]) * 0.5 / 4. / den / 2.;
+  }
+}
+
+template <class R>
+static hipError_t launch_sweep_prep(const los_params *P, hipStream_t s)
+{
+  const long n = (long)P->F * P->ntau;
+  if (n == 0) return hipSuccess;
+  const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
+  hipLaunchKernelGGL(los_sweep_prep<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *P, NL);
+  return hipGetLastError();
+}
+
+// P->tiles / P->xcd_tile must describe tiles of at most NK positions.
+template <class Pol, int NK>
+static hipError_t launch_sweep(const los_params *P, hipStream_t s)
+{
+  if (P->n_pos == 0 || P->n_wave == 0) return hipSuccess;
+  int most = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int n = P->xcd_tile[x + 1] - P->xcd_tile[x];
+    most = n > most ? n : most;
+  }
+  if (most == 0) return hipSuccess;
+  const dim3 grid(8 * most, P->n_wave);
+  const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
+  switch (NL) {
+#define LOS_SW(n) \
+  case n: hipLaunchKernelGGL((los_sweep<Pol, NK, n>), grid, dim3(64), 0, s, *P); break;
+    LOS_SW(1) LOS_SW(2) LOS_SW(3) LOS_SW(4) LOS_SW(5) LOS_SW(6) LOS_SW(7) LOS_SW(8)
+#undef LOS_SW
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}

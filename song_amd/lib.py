@@ -92,11 +92,15 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_destroy": (None, [vp]),
         "los_engine_set_form": (i, [vp, i, ch]),
         "los_engine_load": (i, [vp] * 7 + [i, i, ch]),
+        "los_engine_load_pairs": (i, [vp] * 7 + [C.POINTER(i), C.POINTER(i), i, ch]),
+        "los_engine_autotune": (i, [vp, i, ch]),
+        "los_engine_set_shape": (i, [vp, i, i, i, i, i, ch]),
+        "los_engine_get_shape": (None, [vp, i] + [C.POINTER(i)] * 4),
+        "los_engine_capture": (i, [vp, ch]),
         "los_engine_run": (i, [vp, C.POINTER(C.c_float), ch]),
         "los_engine_last_kernel_ms": (i, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "los_engine_store": (i, [vp, vp, ch]),
         "los_engine_n_elements": (C.c_long, [vp]),
-        "los_engine_live_terms": (d, [vp]),
         "los_engine_alg_bytes": (d, [vp]),
         "los_engine_stream": (vp, [vp]),
     }
@@ -229,6 +233,24 @@ class Scenario:
         if rc != SUCCESS:
             raise LosError("transfer2_init failed: " + self.error_message())
 
+    def prepare(self) -> None:
+        """transfer2_init up to the integral: indices, k3 sizes, all k1 levels allocated."""
+        rc = self.lib.los_transfer2_prepare(*self.structs())
+        self._inited = True
+        if rc != SUCCESS:
+            raise LosError("los_transfer2_prepare failed: " + self.error_message())
+
+    def pair_offsets(self):
+        """{(i1, i2): (offset, k3 count)} of each pair inside one transfer type of gather()."""
+        sizes = self.k3_sizes()[:, 0]
+        out, o, q = {}, 0, 0
+        for i1 in range(self.cfg.k_size):
+            for i2 in range(i1 + 1):
+                out[(i1, i2)] = (o, int(sizes[q]))
+                o += int(sizes[q])
+                q += 1
+        return out
+
     def error_message(self) -> str:
         # ptr2->error_message lives inside struct transfers2; read it via offset
         return _transfers2_error(self.s["ptr2"])
@@ -313,6 +335,82 @@ class Scenario:
         if self.ptr:
             self.free_transfers()
             self.lib.los_scenario_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """The LOS engine of one HIP device (los_engine_* of include/song_los.h)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.ptr = C.c_void_p()
+        err = _err()
+        check(self.lib.los_engine_create(device, C.byref(self.ptr), err), err, "los_engine_create")
+
+    def _call(self, name, *args):
+        err = _err()
+        check(getattr(self.lib, name)(self.ptr, *args, err), err, name)
+
+    def set_form(self, form: int) -> None:
+        self._call("los_engine_set_form", form)
+
+    def load(self, scen: "Scenario", k1_begin: int = 0, k1_end: int | None = None) -> None:
+        s = scen.s
+        k1_end = scen.cfg.k_size if k1_end is None else k1_end
+        self._call("los_engine_load", s["ppr"], s["ppr2"], s["ppt2"], s["pbs"], s["pbs2"],
+                   s["ptr2"], k1_begin, k1_end)
+
+    def load_pairs(self, scen: "Scenario", pairs) -> None:
+        s = scen.s
+        k1 = (C.c_int * len(pairs))(*[int(p[0]) for p in pairs])
+        k2 = (C.c_int * len(pairs))(*[int(p[1]) for p in pairs])
+        self._call("los_engine_load_pairs", s["ppr"], s["ppr2"], s["ppt2"], s["pbs"], s["pbs2"],
+                   s["ptr2"], k1, k2, len(pairs))
+
+    def run(self) -> float:
+        ms = C.c_float()
+        self._call("los_engine_run", C.byref(ms))
+        return ms.value
+
+    def last_kernel_ms(self):
+        a, b = C.c_float(), C.c_float()
+        self.lib.los_engine_last_kernel_ms(self.ptr, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def autotune(self, reps: int = 2) -> None:
+        self._call("los_engine_autotune", reps)
+
+    def set_shape(self, fp32: int, kind: int, nk: int, ch: int = 0, persistent: int = 0) -> None:
+        """kind 0: stream kernel (nk, ch, persistent); kind 1: sweep kernel (nk)."""
+        self._call("los_engine_set_shape", fp32, kind, nk, ch, persistent)
+
+    def get_shape(self, fp32: int):
+        """(kind, nk, ch, persistent) of the form's kernel."""
+        v = [C.c_int() for _ in range(4)]
+        self.lib.los_engine_get_shape(self.ptr, fp32, *[C.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def capture(self) -> None:
+        self._call("los_engine_capture")
+
+    def store(self, scen: "Scenario") -> None:
+        self._call("los_engine_store", scen.s["ptr2"])
+
+    def n_elements(self) -> int:
+        return self.lib.los_engine_n_elements(self.ptr)
+
+    def alg_bytes(self) -> float:
+        return self.lib.los_engine_alg_bytes(self.ptr)
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.los_engine_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):

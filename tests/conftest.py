@@ -54,3 +54,12 @@ def run_oracle(scen, oracle):
     out = scen.gather()
     scen.free_transfers(lambda a, b, c: oracle.los_ref_transfer2_free(a, b, c))
     return out
+
+
+def field_errors(got, ref, tt2_size):
+    """|got - ref| / max|ref| over each transfer type's whole (l,m) field."""
+    import numpy as np
+
+    g, r = np.asarray(got).reshape(tt2_size, -1), np.asarray(ref).reshape(tt2_size, -1)
+    sc = np.abs(r).max(axis=1, keepdims=True)
+    return np.divide(np.abs(g - r), sc, out=np.zeros_like(g), where=sc > 0)

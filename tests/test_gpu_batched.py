@@ -1,0 +1,135 @@
+"""Batched form of the LOS engine on cuda:0.
+
+* every tile shape of the FMA-FP64 and FP32-compensated kernels (dispatch and
+  persistent work-queue forms) against the oracle;
+* explicit pair lists in any order: each pair's k3 row equals the oracle's bit for bit
+  (plain FP64 form), whatever else is in the batch;
+* uneven triangle-limited k3 counts (8 to several hundred per pair) in one batch;
+* hipGraph capture: replays, interleaved with eager runs, equal the eager result.
+"""
+import numpy as np
+import pytest
+
+from song_amd import lib as L
+from tests.conftest import field_errors, run_oracle
+from tests.test_gpu_parity import CASES
+
+SHAPES = ([(0, nk, ch, p) for nk in (2, 4, 8) for ch in (16, 32) for p in (0, 1)] +
+          [(1, nk, 0, 0) for nk in (1, 2, 4)])
+
+# log k in [1e-5, 0.2], 8 coarse k3 per pair, fine step 5e-4: fine k3 counts from 8
+# (narrow triangles, coarse grid only) to about 400 (k1 = k2 = 0.2)
+UNEVEN = dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=6, l_max_los_p=6, k_size=10,
+              k_min=1e-5, k_max=0.2, k3_size=8, tau_size=160, l_size=12, l_max=2000,
+              dk_fine=5e-4, xx_max=6000.0)
+
+
+def _run(eng, scen, form):
+    eng.set_form(form)
+    eng.run()
+    eng.store(scen)
+    return scen.gather()
+
+
+def _tt2(scen, flat):
+    return flat.size // int(scen.k3_sizes()[:, 0].sum())
+
+
+@pytest.mark.gpu
+def test_every_tile_shape(oracle):
+    cfg = L.Config(**CASES["TEB_m012"])
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    s = L.Scenario(cfg)
+    s.prepare()
+    eng = L.Engine(0)
+    eng.load(s)
+    assert np.array_equal(_run(eng, s, L.FORM_FP64), ref)
+    tt2 = _tt2(s, ref)
+    for form, fp32, tol in ((L.FORM_FP64_FAST, 0, 1e-12), (L.FORM_FP32C, 1, 1e-5)):
+        eng.set_form(form)
+        for sh in SHAPES:
+            eng.set_shape(fp32, *sh)
+            print(f"form {form} shape {sh}", flush=True)
+            err = field_errors(_run(eng, s, form), ref, tt2)
+            assert err.max() < tol, f"form {form} shape {sh}: {err.max():.3e}"
+    eng.close()
+    s.close()
+
+
+@pytest.mark.gpu
+def test_pair_list_any_order(oracle):
+    cfg = L.Config(**CASES["TEB_m012"])
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    s = L.Scenario(cfg)
+    s.prepare()
+    offs = s.pair_offsets()
+    tt2 = _tt2(s, ref)
+    per_tt = ref.size // tt2
+    pairs = list(offs)
+    rng = np.random.default_rng(7)
+    eng = L.Engine(0)
+    for trial in range(3):
+        pick = [pairs[i] for i in rng.permutation(len(pairs))[: 3 + 4 * trial]]
+        print(f"pairs {pick}", flush=True)
+        eng.load_pairs(s, pick)
+        got = _run(eng, s, L.FORM_FP64)
+        for p in pick:
+            o, n = offs[p]
+            for tt in range(tt2):
+                a = tt * per_tt + o
+                assert np.array_equal(got[a:a + n], ref[a:a + n]), f"pair {p} tt {tt}"
+    eng.close()
+    s.close()
+
+
+@pytest.mark.gpu
+def test_uneven_k3_counts(oracle):
+    cfg = L.Config(**UNEVEN)
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    s = L.Scenario(cfg)
+    s.prepare()
+    sizes = s.k3_sizes()[:, 0]
+    print(f"fine k3 per pair: min {sizes.min()} max {sizes.max()}", flush=True)
+    assert sizes.min() <= 16 and sizes.max() >= 300
+    eng = L.Engine(0)
+    eng.load(s)
+    assert np.array_equal(_run(eng, s, L.FORM_FP64), ref)
+    tt2 = _tt2(s, ref)
+    for form, fp32, tol in ((L.FORM_FP64_FAST, 0, 1e-12), (L.FORM_FP32C, 1, 1e-5)):
+        eng.set_form(form)
+        eng.autotune(1)
+        print(f"form {form} tuned shape {eng.get_shape(fp32)}", flush=True)
+        err = field_errors(_run(eng, s, form), ref, tt2)
+        assert err.max() < tol, f"form {form}: {err.max():.3e}"
+        for sh in ((0, 4, 32, 1), (1, 2, 0, 0)):  # persistent work queue; sweep
+            eng.set_shape(fp32, *sh)
+            err = field_errors(_run(eng, s, form), ref, tt2)
+            assert err.max() < tol, f"form {form} shape {sh}: {err.max():.3e}"
+    eng.close()
+    s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form,shape", [(L.FORM_FP64, None), (L.FORM_FP32C, (0, 4, 32, 1)),
+                                        (L.FORM_FP64_FAST, (0, 8, 16, 0)),
+                                        (L.FORM_FP32C, (1, 2, 0, 0))])
+def test_graph_replay_matches_eager(form, shape):
+    cfg = L.Config(**CASES["TEB_m012"])
+    s = L.Scenario(cfg)
+    s.prepare()
+    eng = L.Engine(0)
+    eng.load(s)
+    eng.set_form(form)
+    if shape:
+        eng.set_shape(1 if form == L.FORM_FP32C else 0, *shape)
+    eager = _run(eng, s, form)
+    eng.capture()
+    for rep in range(3):
+        got = _run(eng, s, form)
+        assert np.array_equal(got, eager), f"replay {rep}"
+        # eager work of another form between replays
+        other = L.FORM_FP64 if form != L.FORM_FP64 else L.FORM_FP32C
+        _run(eng, s, other)
+        eng.set_form(form)
+    eng.close()
+    s.close()

@@ -82,6 +82,14 @@ def test_fp64_bit_exact_wide_grid(oracle):
     assert np.array_equal(got, ref), f"{np.count_nonzero(got != ref)} of {ref.size} differ"
 
 
+def _field_errors(got, ref, scen):
+    sizes = [int(r[0]) for r in scen.k3_sizes()]
+    tt2 = got.size // sum(sizes)
+    g, r = got.reshape(tt2, -1), ref.reshape(tt2, -1)
+    sc = np.abs(r).max(axis=1, keepdims=True)
+    return np.divide(np.abs(g - r), sc, out=np.zeros_like(g), where=sc > 0)
+
+
 def _row_errors(got, ref, scen):
     sizes = [int(r[0]) for r in scen.k3_sizes()]
     tt2 = got.size // sum(sizes)
@@ -98,10 +106,14 @@ def _row_errors(got, ref, scen):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form,tol", [(L.FORM_FP64_FAST, 1e-10), (L.FORM_FP32C, 1e-5)])
-def test_fast_forms_within_tolerance(form, tol, oracle):
-    """FMA FP64 and FP32-compensated forms against the oracle; error of each element
-    relative to the largest |T| of its (tt, pair) k3 row (T oscillates in k3)."""
+@pytest.mark.parametrize("form,tol_field,tol_row", [(L.FORM_FP64_FAST, 1e-12, 1e-10),
+                                                    (L.FORM_FP32C, 1e-5, 1e-4)])
+def test_fast_forms_within_tolerance(form, tol_field, tol_row, oracle):
+    """FMA FP64 and FP32-compensated forms against the oracle.
+
+    Error of each element relative to the largest |T| of its (l,m) field (all pairs and
+    k3 of one transfer type), and, stricter, relative to the largest |T| of its (tt, pair)
+    k3 row; single elements are not used as scales because T oscillates in k3."""
     cfg = L.Config(**CASES["TEB_m012"])
     ref_s = L.Scenario(cfg)
     ref = run_oracle(ref_s, oracle)
@@ -113,8 +125,11 @@ def test_fast_forms_within_tolerance(form, tol, oracle):
     finally:
         lib.los_set_default_form(L.FORM_FP64)
     got = s.gather()
-    err = _row_errors(got, ref, s)
-    print(f"form {form}: max {err.max():.3e} rms {np.sqrt(np.mean(err**2)):.3e}")
-    assert err.max() < tol
+    field = _field_errors(got, ref, s)
+    row = _row_errors(got, ref, s)
+    print(f"form {form}: field max {field.max():.3e} rms {np.sqrt(np.mean(field**2)):.3e}; "
+          f"row max {row.max():.3e}")
+    assert field.max() < tol_field
+    assert row.max() < tol_row
     s.close()
     ref_s.close()
