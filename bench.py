@@ -47,6 +47,9 @@ WORKLOADS = {
     "c2": _C2,
     "c3": dict(_C2, k_size=64, k_log=2, k_max=0.2, k3_size=8, dk_fine=3.5e-4, xx_max=6000.0),
     "c4": dict(_C2, l_size=128, l_max=3000),
+    # serving: the pairs of the c3 grid with l up to 3000 (jobs pick l_max prefixes)
+    "c5": dict(_C2, k_size=64, k_log=2, k_max=0.2, k3_size=8, dk_fine=3.5e-4, xx_max=6000.0,
+               l_size=128, l_max=3000),
     # small workload for quick checks
     "tiny": dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=4, l_max_los_p=4, k_size=6,
                  k3_size=16, tau_size=128, l_size=12, l_max=400, dk_fine=1e-3, xx_max=1500.0),
@@ -60,6 +63,8 @@ DESCR = {
     "c3": ("batched: 2048 pairs of 64 log-uniform k in [1e-5,0.2], fine k3 8..400 per pair, "
            "T+E+B m=0,1,2 L=0..6, 600 tau, 96 l in [2,2000], one captured graph per step"),
     "c4": "c2 with 128 l in [2,3000]",
+    "c5": ("serving: Poisson jobs of 1-64 pairs of the c3 grid, l_max in {500,1000,2000,3000}, "
+           "T+E+B m=0,1,2, resident tables, HBM-budget admission, 3 slots"),
     "tiny": "small check workload",
 }
 
@@ -154,6 +159,103 @@ def loaded_columns(scen: L.Scenario, pairs):
     return np.concatenate(cols), sizes
 
 
+def serve_bench(args, cfg):
+    """Serving workload: Poisson arrivals at three rates, per form."""
+    import threading
+
+    scen = L.Scenario(cfg)
+    scen.prepare()
+    lm_l = scen.lm_l()
+    fields = 3  # T, E and B (E implies B for m > 0 in this workload)
+    allp = [(i1, i2) for i1 in range(cfg.k_size) for i2 in range(i1 + 1)]
+    offs = scen.pair_offsets()
+    rng = np.random.default_rng(cfg.seed + 5)
+
+    def make_jobs(n):
+        jobs = []
+        for _ in range(n):
+            k = int(rng.integers(1, min(64, len(allp)) + 1))
+            pick = [allp[i] for i in rng.choice(len(allp), k, replace=False)]
+            lmax = int(rng.choice([500, 1000, 2000, 3000]))
+            n_lm = sum(1 for l in lm_l if l <= lmax)
+            jobs.append((pick, lmax, fields * n_lm * sum(offs[p][1] for p in pick)))
+        return jobs
+
+    forms = [args.form] + [f for f in args.also.split(",") if f and f != args.form]
+    out = {}
+    for form in forms:
+        t0 = time.time()
+        srv = L.Server(scen, FORMS[form], budget_bytes=args.serve_budget * 2.0 ** 30, slots=3)
+        print(f"# {form}: server ready in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+        # capacity: a burst admitted together, closed loop
+        burst = make_jobs(48)
+        srv.hold(True)
+        ids = [srv.submit(p, lm) for p, lm, _ in burst]
+        t0 = time.perf_counter()
+        srv.hold(False)
+        for i in ids:
+            srv.wait(i)
+        cap = len(burst) / (time.perf_counter() - t0)
+        res = {"capacity_jobs_per_s": cap, "rates": []}
+        for frac in (0.5, 0.8, 1.1):
+            rate = frac * cap
+            n = int(min(300, max(40, rate * args.serve_seconds)))
+            jobs = make_jobs(n)
+            gaps = rng.exponential(1.0 / rate, n)
+            ids = [None] * n
+
+            def feed():
+                t = time.perf_counter()
+                for q, (p, lm, _) in enumerate(jobs):
+                    t += gaps[q]
+                    d = t - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                    ids[q] = srv.submit(p, lm)
+
+            st0 = srv.stats()
+            th = threading.Thread(target=feed)
+            th.start()
+            lat, t_first, t_last = [], None, 0.0
+            for q in range(n):
+                while ids[q] is None:
+                    time.sleep(1e-4)
+                _, (ts, ta, td) = srv.wait(ids[q])
+                lat.append(1e3 * (td - ts))
+                t_first = ts if t_first is None else min(t_first, ts)
+                t_last = max(t_last, td)
+            th.join()
+            st = srv.stats()
+            el = sum(j[2] for j in jobs)
+            lat = np.array(lat)
+            r = {"offered_jobs_per_s": rate, "jobs": n, "p50_ms": float(np.percentile(lat, 50)),
+                 "p99_ms": float(np.percentile(lat, 99)), "elements_per_s": el / (t_last - t_first),
+                 "launches": st["launches"] - st0["launches"],
+                 "max_busy_slots": st["max_busy_slots"]}
+            print(f"# {form} rate {rate:.1f}/s: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms "
+                  f"{r['elements_per_s']:.3g} el/s", file=sys.stderr, flush=True)
+            res["rates"].append(r)
+        res["resident_bytes"] = srv.stats()["resident_bytes"]
+        res["budget_bytes"] = srv.stats()["budget_bytes"]
+        srv.close()
+        out[form] = res
+    scen.close()
+    head = out[args.form]["rates"][1]
+    line = {
+        "metric": METRIC, "value": head["elements_per_s"], "unit": "elements/s",
+        "n_gpus": 1, "steps": 0, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.form != "fp32c" else "f32-compensated",
+        "data": "synthetic (seeded source model, FP64 J tables built on the host)",
+        "config": {"workload": f"c5: {DESCR['c5']}", "form": args.form,
+                   "value_at": "0.8 x measured capacity"},
+        "latency_ms": {"p50": head["p50_ms"], "p99": head["p99_ms"]},
+        "serving": out,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +272,10 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--errors-out", default="", help="write per-(l,m) errors as JSON here")
+    ap.add_argument("--serve-budget", type=float, default=8.0,
+                    help="c5: HBM budget of the admitted jobs, GiB")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="c5: arrival span per rate, seconds")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py: this build runs the LOS stage on one GPU; --gpus {args.gpus} is not "
@@ -179,6 +285,8 @@ def main(argv=None):
     graph = args.graph == "on" or (args.graph == "auto" and args.workload in GRAPH)
 
     cfg = L.Config(**WORKLOADS[args.workload])
+    if args.workload == "c5":
+        return serve_bench(args, cfg)
     t0 = time.time()
     scen = L.Scenario(cfg)
     scen.prepare()

@@ -109,6 +109,47 @@ double los_engine_alg_bytes(los_engine *eng);
 /* The HIP stream the engine launches on (hipStream_t as void*). */
 void *los_engine_stream(los_engine *eng);
 
+/* ------------------------------------------------------------------------
+ * Serving: a persistent request loop over resident tables.
+ *
+ * los_server_create loads every (k1,k2) pair of the run (sources, coarse and
+ * fine k3 grids, J tables) onto HIP device `device` once and prepares them
+ * (k3 spline, per-(k3,tau) tables of the chosen kernel shape). Jobs are lists
+ * of pairs with an l_max; they are admitted first come first served while the
+ * device bytes of the admitted jobs stay within hbm_budget_bytes, and the jobs
+ * admitted together run as one launch on one of n_slots streams, overlapping
+ * with the other slots. A job's result is [tt][col]: tt runs over the fields
+ * (T, E, B, those present) times the (l,m) offsets with l <= l_max, col over
+ * the fine k3 of its pairs in the order given. Results do not depend on what
+ * else runs: tiles never mix jobs and the kernel shape is fixed.
+ * No counterpart in the reference (SONG runs the stage once per input file).
+ * ---------------------------------------------------------------------- */
+typedef struct los_server los_server;
+int los_server_create(int device, int form, double hbm_budget_bytes, int n_slots,
+                      struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                      struct bessels *pbs, struct bessels2 *pbs2, struct transfers2 *ptr2,
+                      los_server **out, char *errmsg);
+/* Finish the admitted jobs, fail the queued ones, free everything. */
+void los_server_destroy(los_server *srv);
+/* Queue a job; returns its id (>= 0) or -1 with errmsg set (unknown pair, no
+   l <= l_max, or a job larger than the whole budget). Thread-safe. */
+long los_server_submit(los_server *srv, const int *index_k1, const int *index_k2, int n_pairs,
+                       int l_max, char *errmsg);
+/* Pause (hold = 1) or resume (hold = 0) admissions; submitted jobs queue
+   meanwhile and are admitted together on resume. */
+void los_server_hold(los_server *srv, int hold);
+/* Result size of a queued or finished job. */
+int los_server_job_shape(los_server *srv, long job, long *n_values, int *n_lm, int *n_cols);
+/* Wait up to timeout_ms (< 0: no limit) for a job. 0: done, its n_values
+   copied to out and the job released; 1: failed (released) or bad buffer;
+   2: timed out. times (if not NULL) gets the steady-clock seconds of submit,
+   admission and completion. */
+int los_server_wait(los_server *srv, long job, double timeout_ms, double *out, long n,
+                    double *times, char *errmsg);
+/* out[0..5]: jobs done, launches, most slots busy at once, peak admitted
+   bytes, resident bytes, budget. */
+int los_server_stats(los_server *srv, double *out, int n);
+
 /* The part of transfer2_init that precedes the integral: copies tau0, tau_rec
    and rs_rec, runs transfer2_indices_of_transfers (l, m, k3 bookkeeping) and
    allocates every k1 level of ptr2->transfer, so that engine runs can be

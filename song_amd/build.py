@@ -26,6 +26,7 @@ HIP_SOURCES = [
     ("los_engine.hip", []),
     ("los_kernels_exact.hip", ["-ffp-contract=off"]),
     ("los_kernels_fast.hip", []),
+    ("los_serve.hip", []),
 ]
 
 C_FLAGS = ["-O2", "-std=c99", "-D_GNU_SOURCE", "-fPIC", "-fopenmp", "-Wall",

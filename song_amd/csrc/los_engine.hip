@@ -310,9 +310,11 @@ int nk_slot(int nk) { return nk == 1 ? 0 : nk == 2 ? 1 : nk == 4 ? 2 : 3; }
 std::vector<los_tile> make_tiles(const std::vector<int> &job_end, int nk)
 {
   std::vector<los_tile> t;
-  int p = 0;
-  for (int end : job_end)
-    for (; p < end; p += nk) t.push_back(los_tile{p, std::min(nk, end - p)});
+  int begin = 0;
+  for (int end : job_end) {
+    for (int p = begin; p < end; p += nk) t.push_back(los_tile{p, std::min(nk, end - p)});
+    begin = end;
+  }
   return t;
 }
 
@@ -441,6 +443,8 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   for (int i = 0; i < e->F; ++i) forder[i] = i;
   std::stable_sort(forder.begin(), forder.end(), [&](int a, int b) { return kf[a] < kf[b]; });
   e->src_total = src_off;
+  e->hforder = forder;
+  e->hkf = kf;
   e->tt2_size = ptr2->tt2_size;
   e->ntau = ntau;
 
@@ -661,6 +665,38 @@ static int enqueue_los(los_engine *e, const los_shape *sh, char *err, hipEvent_t
   if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
   split_xcd(e->htiles[nk_slot(sh->nk)], e->pos_cost, P.xcd_tile);
   HIPCHK(los_launch_stream(fp32, &P, sh, e->n_cu, e->stream), err);
+  return _SUCCESS_;
+}
+
+// The LOS kernel of the current form on an explicit work list (P carries its
+// forder / pos_job / ocol / jobs / tiles / out; tiles and cost are their host
+// copies); the per-(k3, tau) tables of a sweep shape must already be made.
+int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStream_t s,
+                       const std::vector<los_tile> &tiles, const std::vector<double> &cost,
+                       char *err)
+{
+  if (e->form == LOS_FORM_FP64) {
+    split_xcd(tiles, cost, P->xcd_pos);
+    HIPCHK(los_launch_integrate_exact(P, 0, s), err);
+    return _SUCCESS_;
+  }
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  split_xcd(tiles, cost, P->xcd_tile);
+  if (sh->kind == 1) {
+    P->Jp = fp32 ? (const void *)e->dJp32 : (const void *)e->dJp;
+    HIPCHK(los_launch_sweep(fp32, P, sh->nk, s), err);
+  } else {
+    HIPCHK(los_launch_stream(fp32, P, sh, e->n_cu, s), err);
+  }
+  return _SUCCESS_;
+}
+
+// Per-(k3, tau) tables of the sweep kernels for every loaded fine k3 (waits).
+int make_sweep_tables(los_engine *e, char *err)
+{
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  HIPCHK(los_launch_sweep_prep(fp32, &e->P, e->stream), err);
+  HIPCHK(hipStreamSynchronize(e->stream), err);
   return _SUCCESS_;
 }
 

@@ -85,6 +85,8 @@ struct los_engine {
 
   int n_cu = 256;
   int *dqueue = nullptr;               // work-queue counters of the persistent kernels
+  std::vector<int> hforder;            // work order (host copy)
+  std::vector<double> hkf;             // fine k3 (host copy)
   std::vector<double> pos_cost;        // estimated cost of each sorted position
   std::vector<double> pos_cost_sweep;  // the same for the sweep kernels
   los_shape shape[2] = {{0, 4, 32, 0}, {0, 4, 32, 0}};  // FP64-FMA, FP32C
@@ -135,3 +137,7 @@ int enqueue_prep(los_engine *e, char *err);
 int ensure_j32(los_engine *e, char *err);
 int current_shape(los_engine *e, const los_shape **sh, char *err);
 int ensure_sweep(los_engine *e, const los_shape *sh, char *err);
+int make_sweep_tables(los_engine *e, char *err);
+int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStream_t s,
+                       const std::vector<los_tile> &tiles, const std::vector<double> &cost,
+                       char *err);

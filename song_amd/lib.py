@@ -98,6 +98,14 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_get_shape": (None, [vp, i] + [C.POINTER(i)] * 4),
         "los_engine_capture": (i, [vp, ch]),
         "los_engine_run": (i, [vp, C.POINTER(C.c_float), ch]),
+        "los_server_create": (i, [i, i, d, i] + [vp] * 6 + [C.POINTER(vp), ch]),
+        "los_server_destroy": (None, [vp]),
+        "los_server_submit": (C.c_long, [vp, C.POINTER(i), C.POINTER(i), i, i, ch]),
+        "los_server_job_shape": (i, [vp, C.c_long, C.POINTER(C.c_long), C.POINTER(i),
+                                     C.POINTER(i)]),
+        "los_server_wait": (i, [vp, C.c_long, d, C.POINTER(d), C.c_long, C.POINTER(d), ch]),
+        "los_server_stats": (i, [vp, C.POINTER(d), i]),
+        "los_server_hold": (None, [vp, i]),
         "los_engine_last_kernel_ms": (i, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "los_engine_store": (i, [vp, vp, ch]),
         "los_engine_n_elements": (C.c_long, [vp]),
@@ -239,6 +247,12 @@ class Scenario:
         self._inited = True
         if rc != SUCCESS:
             raise LosError("los_transfer2_prepare failed: " + self.error_message())
+
+    def lm_l(self):
+        """l of each (l,m) offset of the transfer types (l-major, m ascending)."""
+        ls = self.grids()[0]
+        m = list(self.cfg.m_list)
+        return [int(l) for l in ls for mm in m if mm <= l]
 
     def pair_offsets(self):
         """{(i1, i2): (offset, k3 count)} of each pair inside one transfer type of gather()."""
@@ -411,6 +425,71 @@ class Engine:
     def close(self) -> None:
         if self.ptr:
             self.lib.los_engine_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The LOS stage as a service over one scenario's pairs (los_server_* of
+    include/song_los.h)."""
+
+    def __init__(self, scen: "Scenario", form: int, budget_bytes: float, slots: int = 3,
+                 device: int = 0):
+        self.lib = load()
+        self.ptr = C.c_void_p()
+        s = scen.s
+        err = _err()
+        check(self.lib.los_server_create(device, form, float(budget_bytes), slots, s["ppr"],
+                                         s["ppr2"], s["ppt2"], s["pbs"], s["pbs2"], s["ptr2"],
+                                         C.byref(self.ptr), err), err, "los_server_create")
+
+    def submit(self, pairs, l_max: int) -> int:
+        k1 = (C.c_int * len(pairs))(*[int(p[0]) for p in pairs])
+        k2 = (C.c_int * len(pairs))(*[int(p[1]) for p in pairs])
+        err = _err()
+        jid = self.lib.los_server_submit(self.ptr, k1, k2, len(pairs), int(l_max), err)
+        if jid < 0:
+            raise LosError("los_server_submit failed: " + err.value.decode(errors="replace"))
+        return jid
+
+    def shape(self, jid: int):
+        n, nlm, ncol = C.c_long(), C.c_int(), C.c_int()
+        if self.lib.los_server_job_shape(self.ptr, jid, C.byref(n), C.byref(nlm), C.byref(ncol)):
+            raise LosError(f"unknown job {jid}")
+        return n.value, nlm.value, ncol.value
+
+    def wait(self, jid: int, timeout_ms: float = -1.0):
+        """(result [tt][col], (t_submit, t_admit, t_done)) of a job."""
+        n, _, ncol = self.shape(jid)
+        out = np.empty(n, dtype=np.float64)
+        times = (C.c_double * 3)()
+        err = _err()
+        rc = self.lib.los_server_wait(self.ptr, jid, float(timeout_ms),
+                                      out.ctypes.data_as(C.POINTER(C.c_double)), n, times, err)
+        if rc == 2:
+            raise TimeoutError(f"job {jid} not done after {timeout_ms} ms")
+        if rc != 0:
+            raise LosError(err.value.decode(errors="replace"))
+        return out.reshape(-1, ncol), tuple(times)
+
+    def hold(self, on: bool) -> None:
+        self.lib.los_server_hold(self.ptr, int(bool(on)))
+
+    def stats(self):
+        v = (C.c_double * 6)()
+        self.lib.los_server_stats(self.ptr, v, 6)
+        keys = ("jobs_done", "launches", "max_busy_slots", "peak_admitted_bytes",
+                "resident_bytes", "budget_bytes")
+        return dict(zip(keys, list(v)))
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.los_server_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):
