@@ -305,8 +305,7 @@ def main(argv=None):
 
     def run_form(form):
         eng.set_form(FORMS[form])
-        if form != "fp64":
-            eng.autotune(2)
+        eng.autotune(2)
         kern = []
         for _ in range(max(1, args.warmup)):
             eng.run()
@@ -333,9 +332,12 @@ def main(argv=None):
                "prep_ms": prep_ms, "graph": graph, "alg_bytes": eng.alg_bytes()}
         if los_ms > 0:
             res["hbm_gbs"] = res["alg_bytes"] / (los_ms * 1e-3) / 1e9
-        if form != "fp64":
-            kind, nk, ch, pers = eng.get_shape(int(form == "fp32c"))
+        kind, nk, ch, pers = eng.get_shape(FORMS[form])
+        if form == "fp64":
+            res["kernel"] = {0: "workgroup per k3", 2: "k-lane"}[kind]
+        else:
             res["tile_shape"] = ({"kernel": "sweep", "nk": nk} if kind == 1 else
+                                 {"kernel": "k-lane"} if kind == 2 else
                                  {"kernel": "stream", "nk": nk, "ch": ch, "persistent": pers})
         return res, out, tt2
 

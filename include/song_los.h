@@ -72,23 +72,27 @@ int los_engine_load_pairs(los_engine *eng, struct precision *ppr, struct precisi
    Runs the captured graph when los_engine_capture was called for this form. */
 int los_engine_run(los_engine *eng, float *ms, char *errmsg);
 
-/* GPU time of the last eager run's LOS kernel and of its source preparation
-   (re-layout + k3 spline), in ms; -1 after a graph run (only the total is known). */
+/* GPU time of the last eager run's LOS kernel and of everything before it (source
+   re-layout, k3 spline, per-(k3,tau) tables of the kernel shape), in ms; -1 after
+   a graph run (only the total is known). */
 int los_engine_last_kernel_ms(los_engine *eng, float *ms_los, float *ms_spline);
 
-/* Kernel organisation and tile shape of the FMA-FP64 (fp32 = 0) or
-   FP32-compensated (fp32 = 1) form. kind 0, the "stream" kernel: nk fine k3
-   per workgroup, ch time steps per LDS chunk, persistent = 1 for the chip-sized
-   grid fed by the device work queue. kind 1, the "sweep" kernel: one wave per
-   (nk fine k3, m, 64 l) over per-(k3, tau) tables made before it (ch and
-   persistent unused). los_engine_autotune times every shape that fits on the
+/* Kernel organisation and tile shape of a form (enum los_form).
+   Plain FP64 form: kind 0, one workgroup per fine k3 and one lane per (l,m);
+   kind 2, the k-lane kernel below. Both keep the reference's operations.
+   Fast forms: kind 0, the "stream" kernel: nk fine k3 per workgroup, ch time
+   steps per LDS chunk, persistent = 1 for the chip-sized grid fed by the device
+   work queue; kind 1, the "sweep" kernel: one wave per (nk fine k3, m, 64 l)
+   over per-(k3,tau) tables made before it; kind 2, the "k-lane" kernel: one wave
+   per ((l,m), 64 neighbouring fine k3) over per-(k3,tau) tables.
+   los_engine_autotune times every shape of the current form that fits on the
    loaded batch (reps timed launches each) and keeps the fastest; the choice is
    cached per batch shape in the process. Runs tune on first use unless a shape
-   was set after the load. */
+   was set after the load. Arguments a kind does not use are ignored. */
 int los_engine_autotune(los_engine *eng, int reps, char *errmsg);
-int los_engine_set_shape(los_engine *eng, int fp32, int kind, int nk, int ch, int persistent,
+int los_engine_set_shape(los_engine *eng, int form, int kind, int nk, int ch, int persistent,
                          char *errmsg);
-void los_engine_get_shape(los_engine *eng, int fp32, int *kind, int *nk, int *ch, int *persistent);
+void los_engine_get_shape(los_engine *eng, int form, int *kind, int *nk, int *ch, int *persistent);
 
 /* Capture the whole run of the loaded batch (source re-layout, k3 spline, LOS
    kernel with its queue reset) in one hipGraph; later los_engine_run calls of

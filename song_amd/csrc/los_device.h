@@ -37,7 +37,9 @@ typedef struct {
 /* Kernel organisation and tile shape of the fast forms:
    kind 0: "stream" kernel, nk fine k3 per workgroup, ch time steps per LDS
            chunk, persistent = 1 for the chip-sized grid fed by a work queue;
-   kind 1: "sweep" kernel, one wave per (tile of nk fine k3, m, 64 l). */
+   kind 1: "sweep" kernel, one wave per (tile of nk fine k3, m, 64 l);
+   kind 2: "k-lane" kernel, one wave per ((l,m), 64 neighbouring fine k3);
+   kind 0 of the plain FP64 form is its one-workgroup-per-k3 kernel. */
 typedef struct {
   int kind, nk, ch, persistent;
 } los_shape;
@@ -116,6 +118,15 @@ typedef struct {
   void *ixa;             /* per (f, t): index_x and a_J ({int, float} or {int, pad, double}) */
   void *sf;              /* per (f, t, mi, L): S x trapezoid weight of T, E, B (+ pad) */
   const int *tau_end_fw; /* per (f, wave): time steps before every row of the wave is dead */
+
+  /* k-lane kernels (one (l,m) per wave, one fine k3 per lane): tables by rank
+     r = position in the engine's sorted order, k_npad ranks per row */
+  const int *korder;     /* fine index of each rank */
+  const int *rank_of_f;  /* rank of each fine index */
+  long k_npad;
+  void *kixa;            /* [t][r] index_x and a_J */
+  void *ksrc;            /* [tp][t][r] interpolated sources (x trapezoid weight: fast forms) */
+  const void *Jk;        /* J re-laid out [jt][L][lm][x] (FP64 or FP32) */
 
   /* output blocks of the jobs */
   double *out;

@@ -54,6 +54,12 @@ static void free_batch(los_engine *e)
 {
   drop_graph(e);
   dfree(e->dtau_end_fw);
+  dfree(e->drank);
+  if (e->dkixa) (void)hipFree(e->dkixa);
+  if (e->dksrc) (void)hipFree(e->dksrc);
+  e->dkixa = e->dksrc = nullptr;
+  e->klane_form = -1;
+  e->klane_bytes = 0;
   if (e->dixa) (void)hipFree(e->dixa);
   if (e->dsf) (void)hipFree(e->dsf);
   e->dixa = e->dsf = nullptr;
@@ -69,7 +75,7 @@ static void free_batch(los_engine *e)
   dfree(e->dposjob);
   dfree(e->docol);
   dfree(e->djobs);
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < 5; ++i) {
     dfree(e->dtiles[i]);
     e->htiles[i].clear();
   }
@@ -95,6 +101,8 @@ static void free_tables(los_engine *e)
   dfree(e->dmi_of);
   dfree(e->dJp);
   dfree(e->dJp32);
+  dfree(e->dJk);
+  dfree(e->dJk32);
   dfree(e->dxminp);
   dfree(e->dlm_of_lp);
   e->J_owner = nullptr;
@@ -259,6 +267,17 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
   e->m_size = m_size;
   e->n_wave = NLP / 64;
   e->nJp = nJp;
+  {
+    // k-lane layout [jt][L][lm][x]
+    std::vector<double> hJk(nJ);
+    for (size_t r = 0; r < (size_t)J_size * L_size; ++r)
+      for (int x = 0; x < X; ++x)
+        for (int lm = 0; lm < n_lm; ++lm)
+          hJk[(r * n_lm + lm) * X + x] = hJ[(r * X + x) * n_lm + lm];
+    e->nJk = nJ;
+    HIPCHK(hipMalloc((void **)&e->dJk, nJ * sizeof(double)), err);
+    if (upload(e, e->dJk, hJk.data(), nJ * sizeof(double), err)) return _FAILURE_;
+  }
   HIPCHK(hipMalloc((void **)&e->dJp, nJp * sizeof(double)), err);
   if (upload(e, e->dJp, hJp.data(), nJp * sizeof(double), err)) return _FAILURE_;
   if (upload_vec(e, e->dxminp, hxminp, err) || upload_vec(e, e->dlm_of_lp, lm_of_lp, err))
@@ -303,7 +322,7 @@ void split_xcd(const std::vector<los_tile> &tiles, const std::vector<double> &po
   out[8] = nt;
 }
 
-int nk_slot(int nk) { return nk == 1 ? 0 : nk == 2 ? 1 : nk == 4 ? 2 : 3; }
+int nk_slot(int nk) { return nk == 1 ? 0 : nk == 2 ? 1 : nk == 4 ? 2 : nk == 8 ? 3 : 4; }
 
 // Tiles of at most nk consecutive positions that never straddle two jobs
 // (job_end: one past the last position of each job, ascending).
@@ -352,7 +371,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
       return _FAILURE_;
     }
   free_batch(e);
-  e->shape_set[0] = e->shape_set[1] = false;  // shapes are chosen per batch
+  e->shape_set[0] = e->shape_set[1] = e->shape_set[2] = false;  // shapes are chosen per batch
   const int ntau = ppt2->tau_size;
   const int tp2 = ppt2->tp2_size;
 
@@ -502,8 +521,8 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   jobs[0].ttT = ptr2->index_tt2_T;
   jobs[0].ttE = ptr2->index_tt2_E;
   jobs[0].ttB = ptr2->index_tt2_B;
-  for (int i = 0; i < 4; ++i) {
-    e->htiles[i] = make_tiles(std::vector<int>(1, e->F), 1 << i);
+  for (int i = 0; i < 5; ++i) {
+    e->htiles[i] = make_tiles(std::vector<int>(1, e->F), i < 4 ? 1 << i : 64);
     if (upload_vec(e, e->dtiles[i], e->htiles[i], err)) return _FAILURE_;
   }
   if (upload_vec(e, e->dposjob, posjob, err) || upload_vec(e, e->docol, forder, err) ||
@@ -559,6 +578,16 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   P.m_size = e->m_size;
   for (int im = 0; im < LOS_MAX_MI; ++im) P.m_of_mi[im] = im < ppr2->m_size ? ppr2->m[im] : -1;
   P.tau_end_fw = e->dtau_end_fw;
+  {
+    std::vector<int> rank(e->F);
+    for (int sp = 0; sp < e->F; ++sp) rank[forder[sp]] = sp;
+    if (upload_vec(e, e->drank, rank, err)) return _FAILURE_;
+  }
+  P.korder = e->dforder;
+  P.rank_of_f = e->drank;
+  P.k_npad = (e->F + 63) / 64 * 64;
+  P.kixa = e->dkixa;
+  P.ksrc = e->dksrc;
   P.queue = e->dqueue;
   P.src = e->dsrc;
   P.dd = e->ddd;
@@ -637,50 +666,43 @@ int enqueue_prep(los_engine *e, char *err)
   return _SUCCESS_;
 }
 
-// The LOS kernel of the current form with the current tile shape (enqueued).
-// (For a sweep shape, its per-(k3, tau) tables first; `mid`, if given, is
-// recorded between the two so that the sweep kernel can be timed alone.)
-static int enqueue_los(los_engine *e, const los_shape *sh, char *err, hipEvent_t mid = nullptr)
+// Candidate kernel shapes of a form: the plain FP64 form has its
+// one-workgroup-per-k3 kernel (kind 0) and the k-lane kernel (kind 2), both
+// bit-identical to the reference; the fast forms every stream, sweep and
+// k-lane shape that fits.
+static int form_shapes(int form, int tp2_size, los_shape *out, int cap)
 {
-  if (e->form == LOS_FORM_FP64) {
-    if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
-    HIPCHK(los_launch_integrate_exact(&e->P, 0, e->stream), err);
-    return _SUCCESS_;
+  if (form == LOS_FORM_FP64) {
+    int n = 0;
+    if (n < cap) out[n++] = los_shape{0, 1, 0, 0};
+    if (n < cap) out[n++] = los_shape{2, 64, 0, 0};
+    return n;
   }
-  const int fp32 = e->form == LOS_FORM_FP32C;
-  los_params P = e->P;
-  P.tiles = e->dtiles[nk_slot(sh->nk)];
-  if (sh->kind == 1) {
-    if (e->sweep_fp32 != fp32) {
-      snprintf(err, _ERRORMSGSIZE_, "enqueue_los: sweep tables not prepared for this form");
-      return _FAILURE_;
-    }
-    P.Jp = fp32 ? (const void *)e->dJp32 : (const void *)e->dJp;
-    HIPCHK(los_launch_sweep_prep(fp32, &P, e->stream), err);
-    if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
-    split_xcd(e->htiles[nk_slot(sh->nk)], e->pos_cost_sweep, P.xcd_tile);
-    HIPCHK(los_launch_sweep(fp32, &P, sh->nk, e->stream), err);
-    return _SUCCESS_;
-  }
-  if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
-  split_xcd(e->htiles[nk_slot(sh->nk)], e->pos_cost, P.xcd_tile);
-  HIPCHK(los_launch_stream(fp32, &P, sh, e->n_cu, e->stream), err);
-  return _SUCCESS_;
+  return los_shapes(form == LOS_FORM_FP32C, tp2_size, out, cap);
 }
 
-// The LOS kernel of the current form on an explicit work list (P carries its
-// forder / pos_job / ocol / jobs / tiles / out; tiles and cost are their host
-// copies); the per-(k3, tau) tables of a sweep shape must already be made.
+// The LOS kernel of the current form with shape sh, on work list P (tiles and
+// cost: host copies of its tiles and per-position costs). A sweep or k-lane
+// shape's per-(k3, tau) tables must be made already.
 int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStream_t s,
                        const std::vector<los_tile> &tiles, const std::vector<double> &cost,
                        char *err)
 {
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  if (sh->kind == 2) {
+    P->Jk = e->form == LOS_FORM_FP32C ? (const void *)e->dJk32 : (const void *)e->dJk;
+    split_xcd(tiles, cost, P->xcd_tile);
+    if (e->form == LOS_FORM_FP64)
+      HIPCHK(los_launch_klane_exact(P, s), err);
+    else
+      HIPCHK(los_launch_klane(fp32, P, s), err);
+    return _SUCCESS_;
+  }
   if (e->form == LOS_FORM_FP64) {
     split_xcd(tiles, cost, P->xcd_pos);
     HIPCHK(los_launch_integrate_exact(P, 0, s), err);
     return _SUCCESS_;
   }
-  const int fp32 = e->form == LOS_FORM_FP32C;
   split_xcd(tiles, cost, P->xcd_tile);
   if (sh->kind == 1) {
     P->Jp = fp32 ? (const void *)e->dJp32 : (const void *)e->dJp;
@@ -691,11 +713,50 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   return _SUCCESS_;
 }
 
-// Per-(k3, tau) tables of the sweep kernels for every loaded fine k3 (waits).
-int make_sweep_tables(los_engine *e, char *err)
+// Per-(k3, tau) tables of a sweep or k-lane shape for the loaded batch
+// (enqueued on the engine stream).
+static int enqueue_tables(los_engine *e, const los_shape *sh, char *err)
 {
   const int fp32 = e->form == LOS_FORM_FP32C;
-  HIPCHK(los_launch_sweep_prep(fp32, &e->P, e->stream), err);
+  if (sh->kind == 1 && e->form != LOS_FORM_FP64) {
+    if (e->sweep_fp32 != fp32) {
+      snprintf(err, _ERRORMSGSIZE_, "enqueue_tables: sweep tables not allocated for this form");
+      return _FAILURE_;
+    }
+    HIPCHK(los_launch_sweep_prep(fp32, &e->P, e->stream), err);
+  } else if (sh->kind == 2) {
+    if (e->klane_form != e->form) {
+      snprintf(err, _ERRORMSGSIZE_, "enqueue_tables: k-lane tables not allocated for this form");
+      return _FAILURE_;
+    }
+    if (e->form == LOS_FORM_FP64)
+      HIPCHK(los_launch_klane_prep_exact(&e->P, e->stream), err);
+    else
+      HIPCHK(los_launch_klane_prep(fp32, &e->P, e->stream), err);
+  }
+  return _SUCCESS_;
+}
+
+// The LOS stage of the loaded batch with shape sh: its tables (if any), then
+// the kernel. `mid`, if given, is recorded between the two so that the kernel
+// can be timed alone.
+static int enqueue_los(los_engine *e, const los_shape *sh, char *err, hipEvent_t mid = nullptr)
+{
+  if (enqueue_tables(e, sh, err)) return _FAILURE_;
+  if (mid) HIPCHK(hipEventRecord(mid, e->stream), err);
+  los_params P = e->P;
+  const int slot = nk_slot(sh->kind == 2 ? 64 : sh->nk);
+  P.tiles = e->dtiles[slot];
+  const std::vector<double> &cost =
+      sh->kind == 1 ? e->pos_cost_sweep : e->pos_cost;
+  return enqueue_los_params(e, sh, &P, e->stream, e->htiles[slot], cost, err);
+}
+
+// Per-(k3, tau) tables of the current shape for every loaded fine k3 (waits).
+int make_sweep_tables(los_engine *e, char *err)
+{
+  const los_shape *sh = &e->shape[e->form];
+  if (enqueue_tables(e, sh, err)) return _FAILURE_;
   HIPCHK(hipStreamSynchronize(e->stream), err);
   return _SUCCESS_;
 }
@@ -729,6 +790,38 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
   return _SUCCESS_;
 }
 
+// Device tables a k-lane shape needs (allocated outside any capture).
+int ensure_klane(los_engine *e, const los_shape *sh, char *err)
+{
+  if (sh == nullptr || sh->kind != 2) return _SUCCESS_;
+  const int fp32 = e->form == LOS_FORM_FP32C;
+  if (fp32 && e->dJk32 == nullptr) {
+    HIPCHK(hipMalloc((void **)&e->dJk32, std::max<size_t>(1, e->nJk) * sizeof(float)), err);
+    HIPCHK(los_launch_round_j32(e->dJk, e->dJk32, (long)e->nJk, e->stream), err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+  }
+  if (e->klane_form == e->form) return _SUCCESS_;
+  if (e->dkixa) (void)hipFree(e->dkixa);
+  if (e->dksrc) (void)hipFree(e->dksrc);
+  e->dkixa = e->dksrc = nullptr;
+  e->klane_form = -1;
+  const size_t n = (size_t)e->P.k_npad * e->ntau;
+  const size_t ixa_b = std::max<size_t>(1, n) * (fp32 ? 8 : 16);
+  const size_t src_b = std::max<size_t>(1, n * e->P.tp2_size) * (fp32 ? 4 : 8);
+  HIPCHK(hipMalloc(&e->dkixa, ixa_b), err);
+  HIPCHK(hipMalloc(&e->dksrc, src_b), err);
+  e->P.kixa = e->dkixa;
+  e->P.ksrc = e->dksrc;
+  e->klane_form = e->form;
+  e->klane_bytes = ixa_b + src_b;
+  return _SUCCESS_;
+}
+
+static int ensure_tables(los_engine *e, const los_shape *sh, char *err)
+{
+  return ensure_sweep(e, sh, err) || ensure_klane(e, sh, err) ? _FAILURE_ : _SUCCESS_;
+}
+
 int ensure_j32(los_engine *e, char *err)
 {
   if (e->form == LOS_FORM_FP32C && e->dJ32 == nullptr) {
@@ -760,23 +853,22 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
     snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: nothing loaded");
     return _FAILURE_;
   }
-  if (e->form == LOS_FORM_FP64) return _SUCCESS_;  // one kernel shape only
   HIPCHK(hipSetDevice(e->device), err);
-  const int fp32 = e->form == LOS_FORM_FP32C;
+  const int form = e->form;
   const tune_key key = key_of(e);
   {
     std::lock_guard<std::mutex> g(g_tuned_mu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end() && reps >= 0) {
-      e->shape[fp32] = it->second;
-      e->shape_set[fp32] = true;
+      e->shape[form] = it->second;
+      e->shape_set[form] = true;
       return _SUCCESS_;
     }
   }
   if (reps < 1) reps = 1;
   if (ensure_j32(e, err) || enqueue_prep(e, err)) return _FAILURE_;
   los_shape cand[32];
-  const int nc = los_shapes(fp32, e->P.tp2_size, cand, 32);
+  const int nc = form_shapes(form, e->P.tp2_size, cand, 32);
   if (nc == 0) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: no tile shape fits %d source types in LDS",
              e->P.tp2_size);
@@ -785,7 +877,7 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
   float best = 1e30f;
   int ibest = 0;
   for (int i = 0; i < nc; ++i) {
-    if (ensure_sweep(e, &cand[i], err)) return _FAILURE_;
+    if (ensure_tables(e, &cand[i], err)) return _FAILURE_;
     if (enqueue_los(e, &cand[i], err)) return _FAILURE_;  // warm-up
     HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
     for (int r = 0; r < reps; ++r)
@@ -799,50 +891,56 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
       ibest = i;
     }
   }
-  e->shape[fp32] = cand[ibest];
-  e->shape_set[fp32] = true;
+  e->shape[form] = cand[ibest];
+  e->shape_set[form] = true;
   drop_graph(e);
   std::lock_guard<std::mutex> g(g_tuned_mu);
   g_tuned[key] = cand[ibest];
   return _SUCCESS_;
 }
 
-extern "C" int los_engine_set_shape(los_engine *e, int fp32, int kind, int nk, int ch,
+extern "C" int los_engine_set_shape(los_engine *e, int form, int kind, int nk, int ch,
                                     int persistent, char *err)
 {
+  if (form < LOS_FORM_FP64 || form > LOS_FORM_FP32C) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_set_shape: unknown form %d", form);
+    return _FAILURE_;
+  }
   los_shape cand[32];
-  const int nc = los_shapes(fp32 != 0, e->loaded ? e->P.tp2_size : 1, cand, 32);
+  const int nc = form_shapes(form, e->loaded ? e->P.tp2_size : 1, cand, 32);
   for (int i = 0; i < nc; ++i)
-    if (cand[i].kind == kind && cand[i].nk == nk && (kind == 1 || (cand[i].ch == ch &&
-                                                     cand[i].persistent == (persistent != 0)))) {
-      e->shape[fp32 != 0] = cand[i];
-      e->shape_set[fp32 != 0] = true;
+    if (cand[i].kind == kind &&
+        (form == LOS_FORM_FP64 || kind == 2 ||
+         (cand[i].nk == nk &&
+          (kind == 1 || (cand[i].ch == ch && cand[i].persistent == (persistent != 0)))))) {
+      e->shape[form] = cand[i];
+      e->shape_set[form] = true;
       drop_graph(e);
       return _SUCCESS_;
     }
-  snprintf(err, _ERRORMSGSIZE_, "los_engine_set_shape: no tile shape kind=%d nk=%d ch=%d "
-           "persistent=%d for this batch", kind, nk, ch, persistent);
+  snprintf(err, _ERRORMSGSIZE_, "los_engine_set_shape: form %d has no shape kind=%d nk=%d ch=%d "
+           "persistent=%d for this batch", form, kind, nk, ch, persistent);
   return _FAILURE_;
 }
 
-extern "C" void los_engine_get_shape(los_engine *e, int fp32, int *kind, int *nk, int *ch,
+extern "C" void los_engine_get_shape(los_engine *e, int form, int *kind, int *nk, int *ch,
                                      int *persistent)
 {
-  const los_shape &s = e->shape[fp32 != 0];
+  if (form < LOS_FORM_FP64 || form > LOS_FORM_FP32C) form = LOS_FORM_FP64;
+  const los_shape &s = e->shape[form];
   if (kind) *kind = s.kind;
   if (nk) *nk = s.nk;
   if (ch) *ch = s.ch;
   if (persistent) *persistent = s.persistent;
 }
 
-// Shape of the current form, tuned on first use.
+// Shape of the current form, tuned on first use, with its tables allocated.
 int current_shape(los_engine *e, const los_shape **sh, char *err)
 {
-  const int fp32 = e->form == LOS_FORM_FP32C;
-  if (e->form != LOS_FORM_FP64 && !e->shape_set[fp32])
+  if (!e->shape_set[e->form])
     if (los_engine_autotune(e, 2, err)) return _FAILURE_;
-  *sh = &e->shape[fp32];
-  return ensure_sweep(e, *sh, err);
+  *sh = &e->shape[e->form];
+  return ensure_tables(e, *sh, err);
 }
 
 extern "C" int los_engine_capture(los_engine *e, char *err)
@@ -893,10 +991,11 @@ extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
     if (ms) *ms = t;
     return _SUCCESS_;
   }
+  // ev0 .. ev1: source re-layout, k3 spline and the shape's per-(k3, tau)
+  // tables; ev1 .. ev2: the LOS kernel alone
   HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
   if (enqueue_prep(e, err)) return _FAILURE_;
-  HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
-  if (enqueue_los(e, sh, err)) return _FAILURE_;
+  if (enqueue_los(e, sh, err, e->ev[1])) return _FAILURE_;
   HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
   HIPCHK(hipEventSynchronize(e->ev[2]), err);
   HIPCHK(hipEventElapsedTime(&e->ms_spline, e->ev[0], e->ev[1]), err);

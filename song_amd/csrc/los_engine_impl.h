@@ -25,6 +25,10 @@ extern "C" hipError_t los_stream_set_attrs(int tp2_size);
 extern "C" int los_shapes(int fp32, int tp2_size, los_shape *out, int cap);
 extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s);
+extern "C" hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_klane(int fp32, const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_klane_prep_exact(const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_klane_exact(const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_round_j32(const double *src, float *dst, long n, hipStream_t s);
 
 struct los_engine {
@@ -52,6 +56,10 @@ struct los_engine {
   int *dxminp = nullptr, *dlm_of_lp = nullptr;
   int n_lp = 0, n_wave = 0, m_size = 0;
   std::vector<int> wave_minx;  // smallest first-live x index over the lanes of each wave class
+  // and for the k-lane kernels: [jt][L][lm][x]
+  double *dJk = nullptr;
+  float *dJk32 = nullptr;
+  size_t nJk = 0;
   double J_window_bytes = 0;
 
   // time grid
@@ -65,8 +73,8 @@ struct los_engine {
   int *dfpair = nullptr, *dfsrc = nullptr, *dfbr = nullptr, *dforder = nullptr;
   int *dposjob = nullptr, *docol = nullptr;
   los_job *djobs = nullptr;
-  std::vector<los_tile> htiles[4];     // tiles of 1, 2, 4, 8 positions
-  los_tile *dtiles[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<los_tile> htiles[5];     // tiles of 1, 2, 4, 8, 64 positions
+  los_tile *dtiles[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double *draw = nullptr, *dsrc = nullptr, *ddd = nullptr, *dout = nullptr;
   long src_total = 0;
   int nc_max = 0;
@@ -82,6 +90,10 @@ struct los_engine {
   void *dixa = nullptr, *dsf = nullptr;  // per-(k3, tau) tables of the sweep kernels
   int sweep_fp32 = -1;                   // precision they hold (-1: none)
   size_t sweep_bytes = 0;
+  int *drank = nullptr;                  // rank (sorted position) of each fine index
+  void *dkixa = nullptr, *dksrc = nullptr;  // tables of the k-lane kernels
+  int klane_form = -1;                   // form whose tables they hold (-1: none)
+  size_t klane_bytes = 0;
 
   int n_cu = 256;
   int *dqueue = nullptr;               // work-queue counters of the persistent kernels
@@ -89,8 +101,9 @@ struct los_engine {
   std::vector<double> hkf;             // fine k3 (host copy)
   std::vector<double> pos_cost;        // estimated cost of each sorted position
   std::vector<double> pos_cost_sweep;  // the same for the sweep kernels
-  los_shape shape[2] = {{0, 4, 32, 0}, {0, 4, 32, 0}};  // FP64-FMA, FP32C
-  bool shape_set[2] = {false, false};  // chosen by the autotuner or the caller
+  // kernel kind and shape per form (plain FP64, FP64-FMA, FP32C)
+  los_shape shape[3] = {{0, 1, 0, 0}, {0, 4, 32, 0}, {0, 4, 32, 0}};
+  bool shape_set[3] = {false, false, false};  // chosen by the autotuner or the caller
 
   // captured run (transpose, spline, LOS) of the loaded batch
   hipGraph_t graph = nullptr;
@@ -138,6 +151,8 @@ int ensure_j32(los_engine *e, char *err);
 int current_shape(los_engine *e, const los_shape **sh, char *err);
 int ensure_sweep(los_engine *e, const los_shape *sh, char *err);
 int make_sweep_tables(los_engine *e, char *err);
+int make_klane_tables(los_engine *e, char *err);
+int ensure_klane(los_engine *e, const los_shape *sh, char *err);
 int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStream_t s,
                        const std::vector<los_tile> &tiles, const std::vector<double> &cost,
                        char *err);

@@ -17,6 +17,7 @@
 #include <climits>
 
 #include "los_device.h"
+#include "los_klane.h"
 
 extern "C" {
 
@@ -379,6 +380,28 @@ hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double
   const long grid = (ncols + block - 1) / block;
   hipLaunchKernelGGL(los_spline_k3_tau, dim3((unsigned)grid), dim3(block), 0, s, pairs, ncols, kc,
                      spl, src, dd, ntau, tp2);
+  return hipGetLastError();
+}
+
+// k-lane organisation of the plain FP64 form (same per-element operations).
+hipError_t los_launch_klane_prep_exact(const los_params *P, hipStream_t s)
+{
+  return launch_klane_prep<double, false>(P, s);
+}
+
+hipError_t los_launch_klane_exact(const los_params *P, hipStream_t s)
+{
+  if (P->n_pos == 0 || P->n_lm_max == 0) return hipSuccess;
+  const dim3 grid = klane_grid(P);
+  if (grid.x == 0) return hipSuccess;
+  const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
+  switch (NL) {
+#define LOS_KLX(n) \
+  case n: hipLaunchKernelGGL((los_klane_exact<n>), grid, dim3(64), 0, s, *P); break;
+    LOS_KLX(1) LOS_KLX(2) LOS_KLX(3) LOS_KLX(4) LOS_KLX(5) LOS_KLX(6) LOS_KLX(7) LOS_KLX(8)
+#undef LOS_KLX
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

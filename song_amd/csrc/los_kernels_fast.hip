@@ -5,6 +5,7 @@
 
 #include "los_stream.h"
 #include "los_sweep.h"
+#include "los_klane.h"
 
 __global__ void __launch_bounds__(256) los_round_j32(const double *src, float *dst, long n)
 {
@@ -76,6 +77,33 @@ hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s
   }
 }
 
+// k-lane organisation (fast forms): tables then kernel, tiles of 64 positions.
+hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s)
+{
+  return fp32 ? launch_klane_prep<float, true>(P, s) : launch_klane_prep<double, true>(P, s);
+}
+
+hipError_t los_launch_klane(int fp32, const los_params *P, hipStream_t s)
+{
+  if (P->n_pos == 0 || P->n_lm_max == 0) return hipSuccess;
+  const dim3 grid = klane_grid(P);
+  if (grid.x == 0) return hipSuccess;
+  const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
+  switch (NL) {
+#define LOS_KL(n)                                                                        \
+  case n:                                                                                \
+    if (fp32)                                                                            \
+      hipLaunchKernelGGL((los_klane<StreamFP32C, n>), grid, dim3(64), 0, s, *P);         \
+    else                                                                                 \
+      hipLaunchKernelGGL((los_klane<StreamFP64, n>), grid, dim3(64), 0, s, *P);          \
+    break;
+    LOS_KL(1) LOS_KL(2) LOS_KL(3) LOS_KL(4) LOS_KL(5) LOS_KL(6) LOS_KL(7) LOS_KL(8)
+#undef LOS_KL
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // Candidate shapes: the stream shapes (both dispatch orders) that fit the LDS of
 // one workgroup, then the sweep shapes.
 int los_shapes(int fp32, int tp2_size, los_shape *out, int cap)
@@ -88,6 +116,7 @@ int los_shapes(int fp32, int tp2_size, los_shape *out, int cap)
   LOS_SHAPES(LOS_LIST)
 #undef LOS_LIST
   for (int nk = 1; nk <= 4 && n < cap; nk *= 2) out[n++] = los_shape{1, nk, 0, 0};
+  if (n < cap) out[n++] = los_shape{2, 64, 0, 0};
   return n;
 }
 

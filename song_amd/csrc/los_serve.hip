@@ -157,7 +157,7 @@ static int launch_batch(los_server *S, slot_rec &sl, const std::vector<long> &id
     }
     job_end.push_back((int)forder.size());
   }
-  const int nk = e->form == LOS_FORM_FP64 ? 1 : S->shape.nk;
+  const int nk = S->shape.kind == 2 ? 64 : S->shape.nk;
   std::vector<los_tile> tiles = make_tiles(job_end, nk);
 
   // descriptors, packed into one upload
@@ -307,7 +307,7 @@ extern "C" int los_server_create(int device, int form, double hbm_budget_bytes, 
   }
   if (rc == _SUCCESS_) rc = ensure_j32(e, err);
   if (rc == _SUCCESS_) rc = current_shape(e, &sh, err);
-  if (rc == _SUCCESS_ && form != LOS_FORM_FP64 && sh->kind == 1) rc = make_sweep_tables(e, err);
+  if (rc == _SUCCESS_) rc = make_sweep_tables(e, err);  // tables of a sweep or k-lane shape
   if (rc != _SUCCESS_) {
     los_engine_destroy(e);
     delete S;
@@ -329,11 +329,12 @@ extern "C" int los_server_create(int device, int form, double hbm_budget_bytes, 
     for (int im = 0; im <= ppr2->index_m_max[ptr2->l[il]]; ++im)
       S->lm_l[ptr2->lm_array[il][im]] = ptr2->l[il];
   const std::vector<double> &pc =
-      (form != LOS_FORM_FP64 && S->shape.kind == 1) ? e->pos_cost_sweep : e->pos_cost;
+      S->shape.kind == 1 ? e->pos_cost_sweep : e->pos_cost;
   S->fcost.assign(e->F, 0.0);
   for (int sp = 0; sp < e->F; ++sp) S->fcost[e->hforder[sp]] = pc[sp];
-  S->resident = (size_t)(16.0 * e->src_total) + e->nJ * 8 + e->nJp * 8 +
-                (e->dJp32 ? e->nJp * 4 : 0) + (e->dJ32 ? e->nJ * 4 : 0) + e->sweep_bytes;
+  S->resident = (size_t)(16.0 * e->src_total) + e->nJ * 8 + e->nJp * 8 + e->nJk * 8 +
+                (e->dJp32 ? e->nJp * 4 : 0) + (e->dJ32 ? e->nJ * 4 : 0) +
+                (e->dJk32 ? e->nJk * 4 : 0) + e->sweep_bytes + e->klane_bytes;
 
   S->slots.resize(n_slots);
   for (slot_rec &sl : S->slots) {

@@ -400,14 +400,16 @@ class Engine:
     def autotune(self, reps: int = 2) -> None:
         self._call("los_engine_autotune", reps)
 
-    def set_shape(self, fp32: int, kind: int, nk: int, ch: int = 0, persistent: int = 0) -> None:
-        """kind 0: stream kernel (nk, ch, persistent); kind 1: sweep kernel (nk)."""
-        self._call("los_engine_set_shape", fp32, kind, nk, ch, persistent)
+    def set_shape(self, form: int, kind: int, nk: int = 64, ch: int = 0,
+                  persistent: int = 0) -> None:
+        """Kernel of a form. Plain FP64: kind 0 (workgroup per k3) or 2 (k-lane);
+        fast forms: kind 0 stream (nk, ch, persistent), 1 sweep (nk), 2 k-lane."""
+        self._call("los_engine_set_shape", form, kind, nk, ch, persistent)
 
-    def get_shape(self, fp32: int):
+    def get_shape(self, form: int):
         """(kind, nk, ch, persistent) of the form's kernel."""
         v = [C.c_int() for _ in range(4)]
-        self.lib.los_engine_get_shape(self.ptr, fp32, *[C.byref(x) for x in v])
+        self.lib.los_engine_get_shape(self.ptr, form, *[C.byref(x) for x in v])
         return tuple(x.value for x in v)
 
     def capture(self) -> None:

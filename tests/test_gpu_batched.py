@@ -15,7 +15,7 @@ from tests.conftest import field_errors, run_oracle
 from tests.test_gpu_parity import CASES
 
 SHAPES = ([(0, nk, ch, p) for nk in (2, 4, 8) for ch in (16, 32) for p in (0, 1)] +
-          [(1, nk, 0, 0) for nk in (1, 2, 4)])
+          [(1, nk, 0, 0) for nk in (1, 2, 4)] + [(2, 64, 0, 0)])
 
 # log k in [1e-5, 0.2], 8 coarse k3 per pair, fine step 5e-4: fine k3 counts from 8
 # (narrow triangles, coarse grid only) to about 400 (k1 = k2 = 0.2)
@@ -43,12 +43,16 @@ def test_every_tile_shape(oracle):
     s.prepare()
     eng = L.Engine(0)
     eng.load(s)
-    assert np.array_equal(_run(eng, s, L.FORM_FP64), ref)
+    eng.set_form(L.FORM_FP64)
+    for kind in (0, 2):  # both kernels of the plain FP64 form are bit-exact
+        eng.set_shape(L.FORM_FP64, kind)
+        print(f"form 0 kind {kind}", flush=True)
+        assert np.array_equal(_run(eng, s, L.FORM_FP64), ref), f"fp64 kind {kind}"
     tt2 = _tt2(s, ref)
-    for form, fp32, tol in ((L.FORM_FP64_FAST, 0, 1e-12), (L.FORM_FP32C, 1, 1e-5)):
+    for form, tol in ((L.FORM_FP64_FAST, 1e-12), (L.FORM_FP32C, 1e-5)):
         eng.set_form(form)
         for sh in SHAPES:
-            eng.set_shape(fp32, *sh)
+            eng.set_shape(form, *sh)
             print(f"form {form} shape {sh}", flush=True)
             err = field_errors(_run(eng, s, form), ref, tt2)
             assert err.max() < tol, f"form {form} shape {sh}: {err.max():.3e}"
@@ -93,16 +97,19 @@ def test_uneven_k3_counts(oracle):
     assert sizes.min() <= 16 and sizes.max() >= 300
     eng = L.Engine(0)
     eng.load(s)
-    assert np.array_equal(_run(eng, s, L.FORM_FP64), ref)
+    eng.set_form(L.FORM_FP64)
+    for kind in (0, 2):
+        eng.set_shape(L.FORM_FP64, kind)
+        assert np.array_equal(_run(eng, s, L.FORM_FP64), ref), f"fp64 kind {kind}"
     tt2 = _tt2(s, ref)
-    for form, fp32, tol in ((L.FORM_FP64_FAST, 0, 1e-12), (L.FORM_FP32C, 1, 1e-5)):
+    for form, tol in ((L.FORM_FP64_FAST, 1e-12), (L.FORM_FP32C, 1e-5)):
         eng.set_form(form)
         eng.autotune(1)
-        print(f"form {form} tuned shape {eng.get_shape(fp32)}", flush=True)
+        print(f"form {form} tuned shape {eng.get_shape(form)}", flush=True)
         err = field_errors(_run(eng, s, form), ref, tt2)
         assert err.max() < tol, f"form {form}: {err.max():.3e}"
-        for sh in ((0, 4, 32, 1), (1, 2, 0, 0)):  # persistent work queue; sweep
-            eng.set_shape(fp32, *sh)
+        for sh in ((0, 4, 32, 1), (1, 2, 0, 0), (2, 64, 0, 0)):  # work queue; sweep; k-lane
+            eng.set_shape(form, *sh)
             err = field_errors(_run(eng, s, form), ref, tt2)
             assert err.max() < tol, f"form {form} shape {sh}: {err.max():.3e}"
     eng.close()
@@ -110,9 +117,11 @@ def test_uneven_k3_counts(oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form,shape", [(L.FORM_FP64, None), (L.FORM_FP32C, (0, 4, 32, 1)),
+@pytest.mark.parametrize("form,shape", [(L.FORM_FP64, (0, 1)), (L.FORM_FP64, (2, 64)),
+                                        (L.FORM_FP32C, (0, 4, 32, 1)),
                                         (L.FORM_FP64_FAST, (0, 8, 16, 0)),
-                                        (L.FORM_FP32C, (1, 2, 0, 0))])
+                                        (L.FORM_FP32C, (1, 2, 0, 0)),
+                                        (L.FORM_FP32C, (2, 64))])
 def test_graph_replay_matches_eager(form, shape):
     cfg = L.Config(**CASES["TEB_m012"])
     s = L.Scenario(cfg)
@@ -120,8 +129,7 @@ def test_graph_replay_matches_eager(form, shape):
     eng = L.Engine(0)
     eng.load(s)
     eng.set_form(form)
-    if shape:
-        eng.set_shape(1 if form == L.FORM_FP32C else 0, *shape)
+    eng.set_shape(form, *shape)
     eager = _run(eng, s, form)
     eng.capture()
     for rep in range(3):
