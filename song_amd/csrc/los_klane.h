@@ -42,51 +42,83 @@ struct KlaneIXA<double> {
 
 // ---------------------------------------------------------------------------
 // Tables of the k-lane kernels for every loaded fine k3 (rank r = sorted
-// position), one thread per (r, t), r fastest. With WITH_DT the sources are
-// multiplied by the trapezoid weight (fast forms); without, they are the
-// reference's interpolated sources (plain FP64 form, same operations).
+// position). A workgroup of 256 threads makes a tile of 64 ranks x 64 time
+// steps: it reads the sources along tau (their layout) and writes the tables
+// along r through an LDS transpose. With WITH_DT the sources are multiplied
+// by the trapezoid weight (fast forms); without, they are the reference's
+// interpolated sources with its operations (plain FP64 form).
 // ---------------------------------------------------------------------------
 template <class R, bool WITH_DT>
 __global__ void __launch_bounds__(256)
 los_klane_prep(los_params P)
 {
-  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double tile[64][65];
+  __shared__ long r_off[64];
+  __shared__ int r_nc[64], r_br[64];
+  __shared__ double r_k[64], r_aw[64], r_bw[64], r_ca[64], r_cb[64], r_h[64];
+  const int r0 = blockIdx.x * 64, t0 = blockIdx.y * 64;
+  const int tid = threadIdx.x;
   const long npr = P.k_npad;
-  if (id >= npr * P.ntau) return;
-  const int t = (int)(id / npr);
-  const int r = (int)(id - (long)t * npr);
-  if (r >= P.F) return;
-  const int f = P.korder[r];
-  const double k = P.kf[f];
-  const double x = k * P.t0mt[t];
-  const int ix = (int)(x / P.xx_step);
-  KlaneIXA<R> o;
-  o.ix = ix;
-  o.a = (R)((P.xx[ix + 1] - x) / P.xx_step);
-  ((KlaneIXA<R> *)P.kixa)[id] = o;
-
-  const los_pair pr = P.pairs[P.fpair[f]];
-  const int br = P.fbr[f];
-  const double *kc = P.kc + pr.kc_off;
-  const double ks = P.kf[P.fsrc[f]];
-  const double h = kc[br + 1] - kc[br];
-  const double bw = (ks - kc[br]) / h;
-  const double aw = 1 - bw;
-  const double ca = aw * aw * aw - aw;
-  const double cb = bw * bw * bw - bw;
-  const double dt = P.dtau[t];
+  if (tid < 64) {
+    const int r = min(r0 + tid, P.F - 1);
+    const int f = P.korder[r];
+    const los_pair pr = P.pairs[P.fpair[f]];
+    const int br = P.fbr[f];
+    const double *kc = P.kc + pr.kc_off;
+    const double ks = P.kf[P.fsrc[f]];
+    const double h = kc[br + 1] - kc[br];
+    const double bw = (ks - kc[br]) / h;
+    const double aw = 1 - bw;
+    r_off[tid] = pr.src_off;
+    r_nc[tid] = pr.nc;
+    r_br[tid] = br;
+    r_k[tid] = P.kf[f];
+    r_h[tid] = h;
+    r_aw[tid] = aw;
+    r_bw[tid] = bw;
+    r_ca[tid] = aw * aw * aw - aw;
+    r_cb[tid] = bw * bw * bw - bw;
+  }
+  __syncthreads();
+  const int lane = tid & 63, grp = tid >> 6;  // 4 groups of 64
+  // index_x and a_J: lanes along r
+  for (int i = grp; i < 64; i += 4) {
+    const int t = t0 + i, r = r0 + lane;
+    if (t >= P.ntau || r >= P.F) continue;
+    const double x = r_k[lane] * P.t0mt[t];
+    const int ix = (int)(x / P.xx_step);
+    KlaneIXA<R> o;
+    o.ix = ix;
+    o.a = (R)((P.xx[ix + 1] - x) / P.xx_step);
+    ((KlaneIXA<R> *)P.kixa)[(long)t * npr + r] = o;
+  }
   R *out = (R *)P.ksrc;
   for (int tp = 0; tp < P.tp2_size; ++tp) {
-    const long base = pr.src_off + ((long)tp * pr.nc + br) * P.ntau + t;
-    const double *S = P.src + base;
-    double v;
-    if (P.cubic_k3) {
-      const double *D = P.dd + base;
-      v = aw * S[0] + bw * S[P.ntau] + (ca * D[0] + cb * D[P.ntau]) * h * h / 6.0;
-    } else {
-      v = aw * S[0] + bw * S[P.ntau];
+    // read along tau: lane = time step, rows = ranks
+    for (int i = grp; i < 64; i += 4) {
+      const int t = t0 + lane;
+      double v = 0;
+      if (t < P.ntau && r0 + i < P.F) {
+        const long base = r_off[i] + ((long)tp * r_nc[i] + r_br[i]) * P.ntau + t;
+        const double *S = P.src + base;
+        if (P.cubic_k3) {
+          const double *D = P.dd + base;
+          v = r_aw[i] * S[0] + r_bw[i] * S[P.ntau] +
+              (r_ca[i] * D[0] + r_cb[i] * D[P.ntau]) * r_h[i] * r_h[i] / 6.0;
+        } else {
+          v = r_aw[i] * S[0] + r_bw[i] * S[P.ntau];
+        }
+        if (WITH_DT) v = v * P.dtau[t];
+      }
+      tile[i][lane] = v;
     }
-    out[((long)tp * P.ntau + t) * npr + r] = (R)(WITH_DT ? v * dt : v);
+    __syncthreads();
+    // write along r: lane = rank, rows = time steps
+    for (int i = grp; i < 64; i += 4) {
+      const int t = t0 + i, r = r0 + lane;
+      if (t < P.ntau && r < npr) out[((long)tp * P.ntau + t) * npr + r] = (R)tile[lane][i];
+    }
+    __syncthreads();
   }
 }
 
@@ -168,9 +200,13 @@ los_klane(los_params P)
   const KlaneIXA<R> *ixa = (const KlaneIXA<R> *)P.kixa + r;
   const R *sk = (const R *)P.ksrc + r;
   const long tstride = (long)P.ntau * np;
-  const bool doT = P.has_T, doP = P.has_E || P.has_B, mix = P.jEB >= 0;
+  const bool doT = P.has_T && P.jTT >= 0, doP = (P.has_E || P.has_B) && P.jEE >= 0;
   const bool emix = m != 0;
 
+  // Every row's values are loaded at every time step, live or not (rows the
+  // reference skips are zero-filled in the table, their source type is any
+  // valid one), so that a lane has all its loads of a step in flight at once;
+  // the skip rule is then applied as a select.
   A aT = A{}, aE = A{}, aB = A{};
   for (int t0 = 0; t0 < tend; t0 += FOLD) {
     const int t1 = min(tend, t0 + FOLD);
@@ -179,35 +215,40 @@ los_klane(los_params P)
       const KlaneIXA<R> v = ixa[(long)t * np];
       const int ix = v.ix;
       const R a = v.a;
+      const long so = (long)t * np;
+      R tj0[NL], tj1[NL], ts[NL], ej0[NL], ej1[NL], qj0[NL], qj1[NL], se[NL], sb[NL];
 #pragma unroll
       for (int L = 0; L < NL; ++L) {
-        if (m > L) continue;
-        if (doT && L <= P.LmaxT) {
-          const bool live = ix >= xmT[L];
-          if (__ballot(live)) {
-            const R *row = rT + (long)L * rowstep;
-            const R j0 = row[ix], j1 = row[ix + 1];
-            const R s = sk[(long)tpT[L] * tstride + (long)t * np];
-            const R Jv = j1 + a * (j0 - j1);
-            pT += live ? Jv * s : (R)0;
-          }
+        if (doT) {
+          const R *row = rT + (long)L * rowstep;
+          tj0[L] = row[ix];
+          tj1[L] = row[ix + 1];
+          ts[L] = sk[(long)tpT[L] * tstride + so];
         }
-        if (doP && L <= P.LmaxP) {
-          const bool le = ix >= xmE[L], lq = mix && ix >= xmM[L];
-          if (__ballot(le || lq)) {
-            const R *re = rE + (long)L * rowstep;
-            const R *rq = rM + (long)L * rowstep;
-            const R e0 = re[ix], e1 = re[ix + 1];
-            const R q0 = rq[ix], q1 = rq[ix + 1];
-            const R sE = tpE[L] >= 0 && P.has_E ? sk[(long)tpE[L] * tstride + (long)t * np] : (R)0;
-            const R sB = P.has_B ? sk[(long)tpB[L] * tstride + (long)t * np] : (R)0;
-            R Je = e1 + a * (e0 - e1);
-            R Jq = q1 + a * (q0 - q1);
-            Je = le ? Je : (R)0;
-            Jq = lq ? Jq : (R)0;
-            pE += Je * sE + (emix ? Jq * sB : (R)0);
-            pB += Je * sB - Jq * sE;
-          }
+        if (doP) {
+          const R *re = rE + (long)L * rowstep;
+          const R *rq = rM + (long)L * rowstep;
+          ej0[L] = re[ix];
+          ej1[L] = re[ix + 1];
+          qj0[L] = rq[ix];
+          qj1[L] = rq[ix + 1];
+          se[L] = sk[(long)tpE[L] * tstride + so];
+          sb[L] = sk[(long)tpB[L] * tstride + so];
+        }
+      }
+#pragma unroll
+      for (int L = 0; L < NL; ++L) {
+        if (doT) {
+          const R Jv = tj1[L] + a * (tj0[L] - tj1[L]);
+          pT += ix >= xmT[L] ? Jv * ts[L] : (R)0;
+        }
+        if (doP) {
+          R Je = ej1[L] + a * (ej0[L] - ej1[L]);
+          R Jq = qj1[L] + a * (qj0[L] - qj1[L]);
+          Je = ix >= xmE[L] ? Je : (R)0;
+          Jq = ix >= xmM[L] ? Jq : (R)0;
+          pE += Je * se[L] + (emix ? Jq * sb[L] : (R)0);
+          pB += Je * sb[L] - Jq * se[L];
         }
       }
     }
@@ -226,10 +267,9 @@ los_klane(los_params P)
 template <class R, bool WITH_DT>
 static hipError_t launch_klane_prep(const los_params *P, hipStream_t s)
 {
-  const long n = P->k_npad * P->ntau;
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL((los_klane_prep<R, WITH_DT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     s, *P);
+  if (P->F == 0 || P->ntau == 0) return hipSuccess;
+  const dim3 grid((unsigned)(P->k_npad / 64), (unsigned)((P->ntau + 63) / 64));
+  hipLaunchKernelGGL((los_klane_prep<R, WITH_DT>), grid, dim3(256), 0, s, *P);
   return hipGetLastError();
 }
 
@@ -299,49 +339,57 @@ los_klane_exact(los_params P)
   const double *sk = (const double *)P.ksrc + r;
   const long tstride = (long)P.ntau * np;
 
+  // All loads of a time step first (every row, live or not: rows the reference
+  // skips are zero-filled and their source type is any valid one), then the
+  // reference's arithmetic with its skips applied as conditions on the adds.
+  const bool doT = P.has_T && P.jTT >= 0, doP = (P.has_E || P.has_B) && P.jEE >= 0;
   double accT = 0, accEd = 0, accEm = 0, accBd = 0, accBm = 0;
   for (int t = 0; t < tend; ++t) {
     const KlaneIXA<double> v = ixa[(long)t * np];
     const int ix = v.ix;
     const double aJ = v.a;
     const double dt = P.dtau[t];
-    if (P.has_T) {
+    const long so = (long)t * np;
+    double tj0[NL], tj1[NL], ts[NL], ej0[NL], ej1[NL], qj0[NL], qj1[NL], se[NL], sb[NL];
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+      if (doT) {
+        const double *row = rT + (long)L * rowstep;
+        tj0[L] = row[ix];
+        tj1[L] = row[ix + 1];
+        ts[L] = sk[(long)tpT[L] * tstride + so];
+      }
+      if (doP) {
+        const double *re = rE + (long)L * rowstep;
+        const double *rq = rM + (long)L * rowstep;
+        ej0[L] = re[ix];
+        ej1[L] = re[ix + 1];
+        qj0[L] = rq[ix];
+        qj1[L] = rq[ix + 1];
+        se[L] = sk[(long)tpE[L] * tstride + so];
+        sb[L] = sk[(long)tpB[L] * tstride + so];
+      }
+    }
+    if (doT) {
       double integrand = 0;
 #pragma unroll
-      for (int L = 0; L < NL; ++L) {
-        const bool live = ix >= xmT[L];  // INT_MAX for rows the reference skips
-        if (__ballot(live)) {
-          const double *row = rT + (long)L * rowstep;
-          const double j0 = row[ix], j1 = row[ix + 1];
-          const double s = sk[(long)tpT[L] * tstride + (long)t * np];
-          const double Jv = aJ * j0 + (1 - aJ) * j1;
-          if (live) integrand += Jv * s;
-        }
-      }
+      for (int L = 0; L < NL; ++L)
+        if (ix >= xmT[L]) integrand += (aJ * tj0[L] + (1 - aJ) * tj1[L]) * ts[L];
       accT += integrand * dt;
     }
-    if (P.has_E || P.has_B) {
+    if (doP) {
       double iEd = 0, iEm = 0, iBd = 0, iBm = 0;
 #pragma unroll
       for (int L = 0; L < NL; ++L) {
-        const bool le = ix >= xmE[L], lq = ix >= xmM[L];
-        if (__ballot(le || lq)) {
-          const double sE = P.has_E ? sk[(long)tpE[L] * tstride + (long)t * np] : 0.0;
-          const double sB = P.has_B ? sk[(long)tpB[L] * tstride + (long)t * np] : 0.0;
-          const double *re = rE + (long)L * rowstep;
-          const double e0 = re[ix], e1 = re[ix + 1];
-          if (le) {
-            const double Jv = aJ * e0 + (1 - aJ) * e1;
-            if (P.has_E) iEd += Jv * sE;
-            if (P.has_B) iBd += Jv * sB;
-          }
-          const double *rq = rM + (long)L * rowstep;
-          const double q0 = rq[ix], q1 = rq[ix + 1];
-          if (lq) {
-            const double Jv = aJ * q0 + (1 - aJ) * q1;
-            if (m != 0) iEm += Jv * sB;
-            iBm += Jv * sE;
-          }
+        if (ix >= xmE[L]) {
+          const double Jv = aJ * ej0[L] + (1 - aJ) * ej1[L];
+          if (P.has_E) iEd += Jv * se[L];
+          if (P.has_B) iBd += Jv * sb[L];
+        }
+        if (ix >= xmM[L]) {
+          const double Jv = aJ * qj0[L] + (1 - aJ) * qj1[L];
+          if (m != 0) iEm += Jv * sb[L];
+          iBm += Jv * se[L];
         }
       }
       accEd += iEd * dt;
