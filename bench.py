@@ -333,12 +333,10 @@ def main(argv=None):
         if los_ms > 0:
             res["hbm_gbs"] = res["alg_bytes"] / (los_ms * 1e-3) / 1e9
         kind, nk, ch, pers = eng.get_shape(FORMS[form])
-        if form == "fp64":
-            res["kernel"] = {0: "workgroup per k3", 2: "k-lane"}[kind]
-        else:
-            res["tile_shape"] = ({"kernel": "sweep", "nk": nk} if kind == 1 else
-                                 {"kernel": "k-lane"} if kind == 2 else
-                                 {"kernel": "stream", "nk": nk, "ch": ch, "persistent": pers})
+        res["tile_shape"] = ({"kernel": "sweep", "nk": nk} if kind == 1 else
+                             {"kernel": "k-lane", "l_per_lane": ch} if kind == 2 else
+                             {"kernel": "workgroup per k3"} if form == "fp64" else
+                             {"kernel": "stream", "nk": nk, "ch": ch, "persistent": pers})
         return res, out, tt2
 
     forms = [args.form] + [f for f in args.also.split(",") if f and f != args.form]

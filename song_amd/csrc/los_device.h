@@ -127,6 +127,8 @@ typedef struct {
   void *kixa;            /* [t][r] index_x and a_J */
   void *ksrc;            /* [tp][t][r] interpolated sources (x trapezoid weight: fast forms) */
   const void *Jk;        /* J re-laid out [jt][L][lm][x] (FP64 or FP32) */
+  const int *kgrp;       /* lane groups: NLB (l,m) offsets of one m per group, -1 padding */
+  int k_ngrp;            /* number of lane groups */
 
   /* output blocks of the jobs */
   double *out;

@@ -103,6 +103,7 @@ static void free_tables(los_engine *e)
   dfree(e->dJp32);
   dfree(e->dJk);
   dfree(e->dJk32);
+  for (int gs = 0; gs < 3; ++gs) dfree(e->dkgrp[gs]);
   dfree(e->dxminp);
   dfree(e->dlm_of_lp);
   e->J_owner = nullptr;
@@ -277,6 +278,21 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
     e->nJk = nJ;
     HIPCHK(hipMalloc((void **)&e->dJk, nJ * sizeof(double)), err);
     if (upload(e, e->dJk, hJk.data(), nJ * sizeof(double), err)) return _FAILURE_;
+    // lane groups: for each m-list entry, the (l,m) offsets in l order, cut
+    // into groups of 1, 2 or 4
+    for (int gs = 0; gs < 3; ++gs) {
+      const int nlb = 1 << gs;
+      std::vector<int> grp;
+      for (int im = 0; im < m_size; ++im) {
+        std::vector<int> lms;
+        for (int il = 0; il < n_l; ++il)
+          if (im <= ppr2->index_m_max[ptr2->l[il]]) lms.push_back(ptr2->lm_array[il][im]);
+        for (size_t q = 0; q < lms.size(); q += nlb)
+          for (int j = 0; j < nlb; ++j) grp.push_back(q + j < lms.size() ? lms[q + j] : -1);
+      }
+      e->kngrp[gs] = (int)grp.size() / nlb;
+      if (upload_vec(e, e->dkgrp[gs], grp, err)) return _FAILURE_;
+    }
   }
   HIPCHK(hipMalloc((void **)&e->dJp, nJp * sizeof(double)), err);
   if (upload(e, e->dJp, hJp.data(), nJp * sizeof(double), err)) return _FAILURE_;
@@ -675,7 +691,7 @@ static int form_shapes(int form, int tp2_size, los_shape *out, int cap)
   if (form == LOS_FORM_FP64) {
     int n = 0;
     if (n < cap) out[n++] = los_shape{0, 1, 0, 0};
-    if (n < cap) out[n++] = los_shape{2, 64, 0, 0};
+    for (int b = 1; b <= 4 && n < cap; b *= 2) out[n++] = los_shape{2, 64, b, 0};
     return n;
   }
   return los_shapes(form == LOS_FORM_FP32C, tp2_size, out, cap);
@@ -690,12 +706,15 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
 {
   const int fp32 = e->form == LOS_FORM_FP32C;
   if (sh->kind == 2) {
+    const int gs = sh->ch == 4 ? 2 : sh->ch == 2 ? 1 : 0;
     P->Jk = e->form == LOS_FORM_FP32C ? (const void *)e->dJk32 : (const void *)e->dJk;
+    P->kgrp = e->dkgrp[gs];
+    P->k_ngrp = e->kngrp[gs];
     split_xcd(tiles, cost, P->xcd_tile);
     if (e->form == LOS_FORM_FP64)
-      HIPCHK(los_launch_klane_exact(P, s), err);
+      HIPCHK(los_launch_klane_exact(P, sh->ch, s), err);
     else
-      HIPCHK(los_launch_klane(fp32, P, s), err);
+      HIPCHK(los_launch_klane(fp32, P, sh->ch, s), err);
     return _SUCCESS_;
   }
   if (e->form == LOS_FORM_FP64) {
@@ -910,9 +929,10 @@ extern "C" int los_engine_set_shape(los_engine *e, int form, int kind, int nk, i
   const int nc = form_shapes(form, e->loaded ? e->P.tp2_size : 1, cand, 32);
   for (int i = 0; i < nc; ++i)
     if (cand[i].kind == kind &&
-        (form == LOS_FORM_FP64 || kind == 2 ||
-         (cand[i].nk == nk &&
-          (kind == 1 || (cand[i].ch == ch && cand[i].persistent == (persistent != 0)))))) {
+        (kind == 2 ? cand[i].ch == ch
+                   : (form == LOS_FORM_FP64 ||
+                      (cand[i].nk == nk &&
+                       (kind == 1 || (cand[i].ch == ch && cand[i].persistent == (persistent != 0))))))) {
       e->shape[form] = cand[i];
       e->shape_set[form] = true;
       drop_graph(e);

@@ -26,9 +26,9 @@ extern "C" int los_shapes(int fp32, int tp2_size, los_shape *out, int cap);
 extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s);
-extern "C" hipError_t los_launch_klane(int fp32, const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_klane(int fp32, const los_params *P, int nlb, hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep_exact(const los_params *P, hipStream_t s);
-extern "C" hipError_t los_launch_klane_exact(const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_klane_exact(const los_params *P, int nlb, hipStream_t s);
 extern "C" hipError_t los_launch_round_j32(const double *src, float *dst, long n, hipStream_t s);
 
 struct los_engine {
@@ -60,6 +60,8 @@ struct los_engine {
   double *dJk = nullptr;
   float *dJk32 = nullptr;
   size_t nJk = 0;
+  int *dkgrp[3] = {nullptr, nullptr, nullptr};  // lane groups of 1, 2, 4 (l,m) offsets
+  int kngrp[3] = {0, 0, 0};
   double J_window_bytes = 0;
 
   // time grid

@@ -389,17 +389,19 @@ hipError_t los_launch_klane_prep_exact(const los_params *P, hipStream_t s)
   return launch_klane_prep<double, false>(P, s);
 }
 
-hipError_t los_launch_klane_exact(const los_params *P, hipStream_t s)
+hipError_t los_launch_klane_exact(const los_params *P, int nlb, hipStream_t s)
 {
   if (P->n_pos == 0 || P->n_lm_max == 0) return hipSuccess;
   const dim3 grid = klane_grid(P);
   if (grid.x == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  switch (NL) {
-#define LOS_KLX(n) \
-  case n: hipLaunchKernelGGL((los_klane_exact<n>), grid, dim3(64), 0, s, *P); break;
+  switch (NL * 8 + nlb) {
+#define LOS_KLX1(n, b) \
+  case n * 8 + b: hipLaunchKernelGGL((los_klane_exact<n, b>), grid, dim3(64), 0, s, *P); break;
+#define LOS_KLX(n) LOS_KLX1(n, 1) LOS_KLX1(n, 2) LOS_KLX1(n, 4)
     LOS_KLX(1) LOS_KLX(2) LOS_KLX(3) LOS_KLX(4) LOS_KLX(5) LOS_KLX(6) LOS_KLX(7) LOS_KLX(8)
 #undef LOS_KLX
+#undef LOS_KLX1
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -83,22 +83,25 @@ hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s)
   return fp32 ? launch_klane_prep<float, true>(P, s) : launch_klane_prep<double, true>(P, s);
 }
 
-hipError_t los_launch_klane(int fp32, const los_params *P, hipStream_t s)
+hipError_t los_launch_klane(int fp32, const los_params *P, int nlb, hipStream_t s)
 {
   if (P->n_pos == 0 || P->n_lm_max == 0) return hipSuccess;
   const dim3 grid = klane_grid(P);
   if (grid.x == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  switch (NL) {
-#define LOS_KL(n)                                                                        \
-  case n:                                                                                \
+  const int key = NL * 8 + nlb;
+  switch (key) {
+#define LOS_KL1(n, b)                                                                    \
+  case n * 8 + b:                                                                        \
     if (fp32)                                                                            \
-      hipLaunchKernelGGL((los_klane<StreamFP32C, n>), grid, dim3(64), 0, s, *P);         \
+      hipLaunchKernelGGL((los_klane<StreamFP32C, n, b>), grid, dim3(64), 0, s, *P);      \
     else                                                                                 \
-      hipLaunchKernelGGL((los_klane<StreamFP64, n>), grid, dim3(64), 0, s, *P);          \
+      hipLaunchKernelGGL((los_klane<StreamFP64, n, b>), grid, dim3(64), 0, s, *P);       \
     break;
+#define LOS_KL(n) LOS_KL1(n, 1) LOS_KL1(n, 2) LOS_KL1(n, 4)
     LOS_KL(1) LOS_KL(2) LOS_KL(3) LOS_KL(4) LOS_KL(5) LOS_KL(6) LOS_KL(7) LOS_KL(8)
 #undef LOS_KL
+#undef LOS_KL1
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -116,7 +119,7 @@ int los_shapes(int fp32, int tp2_size, los_shape *out, int cap)
   LOS_SHAPES(LOS_LIST)
 #undef LOS_LIST
   for (int nk = 1; nk <= 4 && n < cap; nk *= 2) out[n++] = los_shape{1, nk, 0, 0};
-  if (n < cap) out[n++] = los_shape{2, 64, 0, 0};
+  for (int b = 1; b <= 4 && n < cap; b *= 2) out[n++] = los_shape{2, 64, b, 0};
   return n;
 }
 

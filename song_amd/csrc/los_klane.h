@@ -144,11 +144,84 @@ __device__ __forceinline__ int klane_live_steps(const los_params &P, int r, int 
 }
 
 // ---------------------------------------------------------------------------
-// Fast forms: products in R, partial sums over FOLD time steps in Pol::part_t,
-// folded into Pol::acc_t (TwoSum for the FP32-compensated form).
-// blockIdx.x: XCD-aware tile index; blockIdx.y: (l,m) offset.
+// Common prologue of the k-lane kernels. Workgroup b runs on XCD b % 8;
+// q = b / 8 walks that XCD's tiles, costliest first, with the lane group
+// fastest, so that the waves of one tile run together and share its sources in
+// the XCD's L2. A lane group is NLB (l,m) offsets with the same m (consecutive
+// l): every lane handles one k3 for all of them, so each source value it
+// loads serves NLB elements.
 // ---------------------------------------------------------------------------
-template <class Pol, int NL>
+template <class R, int NL, int NLB>
+struct KlaneWave {
+  int r, sp, m, tend;
+  bool valid;
+  int lm[NLB];
+  bool act[NLB];
+  int xmT[NLB][NL], xmE[NLB][NL], xmM[NLB][NL];
+  int tpT[NL], tpE[NL], tpB[NL];
+  los_tile tl;
+  los_job job;
+
+  // false: nothing to do for this workgroup (uniform)
+  __device__ __forceinline__ bool init(const los_params &P, bool exact_mix_rule)
+  {
+    const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
+    const int tq = q / P.k_ngrp, g = q - tq * P.k_ngrp;
+    if (tq >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return false;
+    tl = P.tiles[P.xcd_tile[xcd + 1] - 1 - tq];
+    job = P.jobs[P.pos_job[tl.pos0]];
+    int first = -1;
+#pragma unroll
+    for (int j = 0; j < NLB; ++j) {
+      lm[j] = P.kgrp[g * NLB + j];
+      act[j] = lm[j] >= 0 && lm[j] < job.n_lm;
+      if (act[j] && first < 0) first = lm[j];
+    }
+    if (first < 0) return false;
+    m = P.m_of[first];
+    const int mi = P.mi_of[first];
+    const int lane = threadIdx.x;
+    valid = lane < tl.npos;
+    sp = tl.pos0 + (valid ? lane : tl.npos - 1);
+    r = P.rank_of_f[P.forder[sp]];
+    int xall = INT_MAX;
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+      tpT[L] = tpE[L] = tpB[L] = 0;
+      const bool rowok = m <= L && L < P.L_size;
+      if (rowok && P.has_T && L <= P.LmaxT) tpT[L] = P.srcT[L][mi];
+      if (rowok && (P.has_E || P.has_B) && L <= P.LmaxP) {
+        if (P.has_E) tpE[L] = P.srcE[L][mi];
+        if (P.has_B) tpB[L] = P.srcB[L][mi];
+      }
+#pragma unroll
+      for (int j = 0; j < NLB; ++j) {
+        xmT[j][L] = xmE[j][L] = xmM[j][L] = INT_MAX;
+        if (!act[j] || !rowok) continue;
+        if (P.has_T && L <= P.LmaxT && P.jTT >= 0)
+          xmT[j][L] = P.xmin[((long)P.jTT * P.L_size + L) * P.n_lm + lm[j]];
+        if ((P.has_E || P.has_B) && L <= P.LmaxP) {
+          if (P.jEE >= 0) xmE[j][L] = P.xmin[((long)P.jEE * P.L_size + L) * P.n_lm + lm[j]];
+          if (P.jEB >= 0 && (!exact_mix_rule || P.has_B))
+            xmM[j][L] = P.xmin[((long)P.jEB * P.L_size + L) * P.n_lm + lm[j]];
+        }
+        xall = min(xall, min(xmT[j][L], min(xmE[j][L], xmM[j][L])));
+      }
+    }
+    tend = klane_wave_max(valid ? klane_live_steps<R>(P, r, xall) : 0);
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Fast forms: products in R, partial sums over FOLD time steps in Pol::part_t,
+// folded into Pol::acc_t (TwoSum for the FP32-compensated form). Every row's
+// values are loaded at every time step, live or not (rows the reference skips
+// are zero-filled in the table, their source type is any valid one), so that
+// a lane has its loads of a step in flight together; the skip rule is applied
+// as a select.
+// ---------------------------------------------------------------------------
+template <class Pol, int NL, int NLB>
 __global__ void __launch_bounds__(64)
 los_klane(los_params P)
 {
@@ -156,112 +229,108 @@ los_klane(los_params P)
   typedef typename Pol::part_t Q;
   typedef typename Pol::acc_t A;
   constexpr int FOLD = 32;
+  KlaneWave<R, NL, NLB> W;
+  if (!W.init(P, false)) return;
 
-  const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
-  if (q >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return;
-  const los_tile tl = P.tiles[P.xcd_tile[xcd + 1] - 1 - q];
-  const los_job job = P.jobs[P.pos_job[tl.pos0]];
-  const int lm = blockIdx.y;
-  if (lm >= job.n_lm) return;
-  const int lane = threadIdx.x;
-  const bool valid = lane < tl.npos;
-  const int sp = tl.pos0 + (valid ? lane : tl.npos - 1);
-  const int r = P.rank_of_f[P.forder[sp]];
-  const int m = P.m_of[lm], mi = P.mi_of[lm];
   const long np = P.k_npad;
-
-  // rows of this (l,m): first live index and source type per L (uniform)
-  int xmT[NL], xmE[NL], xmM[NL], tpT[NL], tpE[NL], tpB[NL];
-  int xall = INT_MAX;
-#pragma unroll
-  for (int L = 0; L < NL; ++L) {
-    xmT[L] = xmE[L] = xmM[L] = INT_MAX;
-    tpT[L] = tpE[L] = tpB[L] = 0;
-    if (m > L || L >= P.L_size) continue;
-    if (P.has_T && L <= P.LmaxT && P.jTT >= 0) {
-      xmT[L] = P.xmin[((long)P.jTT * P.L_size + L) * P.n_lm + lm];
-      tpT[L] = P.srcT[L][mi];
-    }
-    if ((P.has_E || P.has_B) && L <= P.LmaxP) {
-      if (P.jEE >= 0) xmE[L] = P.xmin[((long)P.jEE * P.L_size + L) * P.n_lm + lm];
-      if (P.jEB >= 0) xmM[L] = P.xmin[((long)P.jEB * P.L_size + L) * P.n_lm + lm];
-      if (P.has_E) tpE[L] = P.srcE[L][mi];
-      if (P.has_B) tpB[L] = P.srcB[L][mi];
-    }
-    xall = min(xall, min(xmT[L], min(xmE[L], xmM[L])));
-  }
-  const int tend = klane_wave_max(valid ? klane_live_steps<R>(P, r, xall) : 0);
-
   const R *Jk = (const R *)P.Jk;
   const long rowstep = (long)P.n_lm * P.X;
-  const R *rT = Jk + ((long)P.jTT * P.L_size) * rowstep + (long)lm * P.X;
-  const R *rE = Jk + ((long)P.jEE * P.L_size) * rowstep + (long)lm * P.X;
-  const R *rM = Jk + ((long)(P.jEB >= 0 ? P.jEB : P.jEE) * P.L_size) * rowstep + (long)lm * P.X;
-  const KlaneIXA<R> *ixa = (const KlaneIXA<R> *)P.kixa + r;
-  const R *sk = (const R *)P.ksrc + r;
-  const long tstride = (long)P.ntau * np;
   const bool doT = P.has_T && P.jTT >= 0, doP = (P.has_E || P.has_B) && P.jEE >= 0;
-  const bool emix = m != 0;
+  const int jEB = P.jEB >= 0 ? P.jEB : P.jEE;
+  const KlaneIXA<R> *ixa = (const KlaneIXA<R> *)P.kixa + W.r;
+  const R *sk = (const R *)P.ksrc + W.r;
+  const long tstride = (long)P.ntau * np;
+  const bool emix = W.m != 0;
 
-  // Every row's values are loaded at every time step, live or not (rows the
-  // reference skips are zero-filled in the table, their source type is any
-  // valid one), so that a lane has all its loads of a step in flight at once;
-  // the skip rule is then applied as a select.
-  A aT = A{}, aE = A{}, aB = A{};
-  for (int t0 = 0; t0 < tend; t0 += FOLD) {
-    const int t1 = min(tend, t0 + FOLD);
-    Q pT = 0, pE = 0, pB = 0;
+  A aT[NLB], aE[NLB], aB[NLB];
+#pragma unroll
+  for (int j = 0; j < NLB; ++j) aT[j] = aE[j] = aB[j] = A{};
+  for (int t0 = 0; t0 < W.tend; t0 += FOLD) {
+    const int t1 = min(W.tend, t0 + FOLD);
+    Q pT[NLB], pE[NLB], pB[NLB];
+#pragma unroll
+    for (int j = 0; j < NLB; ++j) pT[j] = pE[j] = pB[j] = 0;
     for (int t = t0; t < t1; ++t) {
       const KlaneIXA<R> v = ixa[(long)t * np];
       const int ix = v.ix;
       const R a = v.a;
       const long so = (long)t * np;
-      R tj0[NL], tj1[NL], ts[NL], ej0[NL], ej1[NL], qj0[NL], qj1[NL], se[NL], sb[NL];
+      R ts[NL], se[NL], sb[NL];
 #pragma unroll
       for (int L = 0; L < NL; ++L) {
-        if (doT) {
-          const R *row = rT + (long)L * rowstep;
-          tj0[L] = row[ix];
-          tj1[L] = row[ix + 1];
-          ts[L] = sk[(long)tpT[L] * tstride + so];
-        }
+        if (doT) ts[L] = sk[(long)W.tpT[L] * tstride + so];
         if (doP) {
-          const R *re = rE + (long)L * rowstep;
-          const R *rq = rM + (long)L * rowstep;
-          ej0[L] = re[ix];
-          ej1[L] = re[ix + 1];
-          qj0[L] = rq[ix];
-          qj1[L] = rq[ix + 1];
-          se[L] = sk[(long)tpE[L] * tstride + so];
-          sb[L] = sk[(long)tpB[L] * tstride + so];
+          se[L] = sk[(long)W.tpE[L] * tstride + so];
+          sb[L] = sk[(long)W.tpB[L] * tstride + so];
         }
       }
 #pragma unroll
-      for (int L = 0; L < NL; ++L) {
-        if (doT) {
-          const R Jv = tj1[L] + a * (tj0[L] - tj1[L]);
-          pT += ix >= xmT[L] ? Jv * ts[L] : (R)0;
+      for (int j = 0; j < NLB; ++j) {
+        if (!W.act[j]) continue;
+        const long lmo = (long)W.lm[j] * P.X + ix;
+        R tj0[NL], tj1[NL], ej0[NL], ej1[NL], qj0[NL], qj1[NL];
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          if (doT) {
+            const R *row = Jk + ((long)P.jTT * P.L_size + L) * rowstep + lmo;
+            tj0[L] = row[0];
+            tj1[L] = row[1];
+          }
+          if (doP) {
+            const R *re = Jk + ((long)P.jEE * P.L_size + L) * rowstep + lmo;
+            const R *rq = Jk + ((long)jEB * P.L_size + L) * rowstep + lmo;
+            ej0[L] = re[0];
+            ej1[L] = re[1];
+            qj0[L] = rq[0];
+            qj1[L] = rq[1];
+          }
         }
-        if (doP) {
-          R Je = ej1[L] + a * (ej0[L] - ej1[L]);
-          R Jq = qj1[L] + a * (qj0[L] - qj1[L]);
-          Je = ix >= xmE[L] ? Je : (R)0;
-          Jq = ix >= xmM[L] ? Jq : (R)0;
-          pE += Je * se[L] + (emix ? Jq * sb[L] : (R)0);
-          pB += Je * sb[L] - Jq * se[L];
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          if (doT) {
+            const R Jv = tj1[L] + a * (tj0[L] - tj1[L]);
+            pT[j] += ix >= W.xmT[j][L] ? Jv * ts[L] : (R)0;
+          }
+          if (doP) {
+            R Je = ej1[L] + a * (ej0[L] - ej1[L]);
+            R Jq = qj1[L] + a * (qj0[L] - qj1[L]);
+            Je = ix >= W.xmE[j][L] ? Je : (R)0;
+            Jq = ix >= W.xmM[j][L] ? Jq : (R)0;
+            pE[j] += Je * se[L] + (emix ? Jq * sb[L] : (R)0);
+            pB[j] += Je * sb[L] - Jq * se[L];
+          }
         }
       }
     }
-    Pol::fold(aT, pT);
-    Pol::fold(aE, pE);
-    Pol::fold(aB, pB);
+#pragma unroll
+    for (int j = 0; j < NLB; ++j) {
+      Pol::fold(aT[j], pT[j]);
+      Pol::fold(aE[j], pE[j]);
+      Pol::fold(aB[j], pB[j]);
+    }
   }
-  if (!valid) return;
-  const double den = 2. * P.l_of[lm] + 1;
-  const long o = job.obase + P.ocol[sp];
-  if (P.has_T) P.out[o + (long)(job.ttT + lm) * job.ostride] = Pol::value(aT) * 0.5 / 4. / den / 2.;
-  if (P.has_E) P.out[o + (long)(job.ttE + lm) * job.ostride] = Pol::value(aE) * 0.5 / 4. / den / 2.;
-  if (P.has_B) P.out[o + (long)(job.ttB + lm) * job.ostride] = Pol::value(aB) * 0.5 / 4. / den / 2.;
+  if (!W.valid) return;
+  const long o = W.job.obase + P.ocol[W.sp];
+#pragma unroll
+  for (int j = 0; j < NLB; ++j) {
+    if (!W.act[j]) continue;
+    const int lm = W.lm[j];
+    const double den = 2. * P.l_of[lm] + 1;
+    if (P.has_T) P.out[o + (long)(W.job.ttT + lm) * W.job.ostride] = Pol::value(aT[j]) * 0.5 / 4. / den / 2.;
+    if (P.has_E) P.out[o + (long)(W.job.ttE + lm) * W.job.ostride] = Pol::value(aE[j]) * 0.5 / 4. / den / 2.;
+    if (P.has_B) P.out[o + (long)(W.job.ttB + lm) * W.job.ostride] = Pol::value(aB[j]) * 0.5 / 4. / den / 2.;
+  }
+}
+
+// grid: 8 x (most tiles of one XCD) x (lane groups)
+static inline dim3 klane_grid(const los_params *P)
+{
+  long most = 0;
+  for (int x = 0; x < 8; ++x) {
+    const long n = P->xcd_tile[x + 1] - P->xcd_tile[x];
+    most = n > most ? n : most;
+  }
+  return dim3((unsigned)(8 * most * P->k_ngrp), 1);
 }
 
 template <class R, bool WITH_DT>
@@ -273,16 +342,7 @@ static hipError_t launch_klane_prep(const los_params *P, hipStream_t s)
   return hipGetLastError();
 }
 
-// grid: (8 * most tiles of one XCD, (l,m) offsets of the largest job)
-static inline dim3 klane_grid(const los_params *P)
-{
-  int most = 0;
-  for (int x = 0; x < 8; ++x) {
-    const int n = P->xcd_tile[x + 1] - P->xcd_tile[x];
-    most = n > most ? n : most;
-  }
-  return dim3(8 * most, P->n_lm_max);
-}
+
 
 // ---------------------------------------------------------------------------
 // Plain FP64 form: the reference's operation order per element (compile this
@@ -291,147 +351,133 @@ static inline dim3 klane_grid(const los_params *P)
 // (a_J J_i + (1 - a_J) J_{i+1}) S; the integral adds integrand x dtau; E adds
 // the B->E mixing integral only for m != 0, B always subtracts the E->B one;
 // then x 1/2, / 4, / (2l+1), / 2 (source/transfer2.c:2259-2329, :1861-2098).
+// Loads are issued for every row, live or not, as in the fast kernel; the
+// reference's skips are conditions on the additions.
 // ---------------------------------------------------------------------------
-template <int NL>
+template <int NL, int NLB>
 __global__ void __launch_bounds__(64)
 los_klane_exact(los_params P)
 {
-  const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
-  if (q >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return;
-  const los_tile tl = P.tiles[P.xcd_tile[xcd + 1] - 1 - q];
-  const los_job job = P.jobs[P.pos_job[tl.pos0]];
-  const int lm = blockIdx.y;
-  if (lm >= job.n_lm) return;
-  const int lane = threadIdx.x;
-  const bool valid = lane < tl.npos;
-  const int sp = tl.pos0 + (valid ? lane : tl.npos - 1);
-  const int r = P.rank_of_f[P.forder[sp]];
-  const int m = P.m_of[lm], mi = P.mi_of[lm];
+  KlaneWave<double, NL, NLB> W;
+  if (!W.init(P, true)) return;
+
   const long np = P.k_npad;
-
-  int xmT[NL], xmE[NL], xmM[NL], tpT[NL], tpE[NL], tpB[NL];
-  int xall = INT_MAX;
-#pragma unroll
-  for (int L = 0; L < NL; ++L) {
-    xmT[L] = xmE[L] = xmM[L] = INT_MAX;
-    tpT[L] = tpE[L] = tpB[L] = 0;
-    if (m > L || L >= P.L_size) continue;
-    if (P.has_T && L <= P.LmaxT) {
-      xmT[L] = P.xmin[((long)P.jTT * P.L_size + L) * P.n_lm + lm];
-      tpT[L] = P.srcT[L][mi];
-    }
-    if ((P.has_E || P.has_B) && L <= P.LmaxP) {
-      xmE[L] = P.xmin[((long)P.jEE * P.L_size + L) * P.n_lm + lm];
-      if (P.has_E) tpE[L] = P.srcE[L][mi];
-      if (P.has_B) tpB[L] = P.srcB[L][mi];
-      if (P.jEB >= 0 && P.has_B) xmM[L] = P.xmin[((long)P.jEB * P.L_size + L) * P.n_lm + lm];
-    }
-    xall = min(xall, min(xmT[L], min(xmE[L], xmM[L])));
-  }
-  const int tend = klane_wave_max(valid ? klane_live_steps<double>(P, r, xall) : 0);
-
   const double *Jk = (const double *)P.Jk;
   const long rowstep = (long)P.n_lm * P.X;
-  const double *rT = Jk + ((long)P.jTT * P.L_size) * rowstep + (long)lm * P.X;
-  const double *rE = Jk + ((long)P.jEE * P.L_size) * rowstep + (long)lm * P.X;
-  const double *rM = Jk + ((long)(P.jEB >= 0 ? P.jEB : P.jEE) * P.L_size) * rowstep + (long)lm * P.X;
-  const KlaneIXA<double> *ixa = (const KlaneIXA<double> *)P.kixa + r;
-  const double *sk = (const double *)P.ksrc + r;
-  const long tstride = (long)P.ntau * np;
-
-  // All loads of a time step first (every row, live or not: rows the reference
-  // skips are zero-filled and their source type is any valid one), then the
-  // reference's arithmetic with its skips applied as conditions on the adds.
   const bool doT = P.has_T && P.jTT >= 0, doP = (P.has_E || P.has_B) && P.jEE >= 0;
-  double accT = 0, accEd = 0, accEm = 0, accBd = 0, accBm = 0;
-  for (int t = 0; t < tend; ++t) {
+  const int jEB = P.jEB >= 0 ? P.jEB : P.jEE;
+  const KlaneIXA<double> *ixa = (const KlaneIXA<double> *)P.kixa + W.r;
+  const double *sk = (const double *)P.ksrc + W.r;
+  const long tstride = (long)P.ntau * np;
+  const int m = W.m;
+
+  double accT[NLB], accEd[NLB], accEm[NLB], accBd[NLB], accBm[NLB];
+#pragma unroll
+  for (int j = 0; j < NLB; ++j) accT[j] = accEd[j] = accEm[j] = accBd[j] = accBm[j] = 0;
+  for (int t = 0; t < W.tend; ++t) {
     const KlaneIXA<double> v = ixa[(long)t * np];
     const int ix = v.ix;
     const double aJ = v.a;
     const double dt = P.dtau[t];
     const long so = (long)t * np;
-    double tj0[NL], tj1[NL], ts[NL], ej0[NL], ej1[NL], qj0[NL], qj1[NL], se[NL], sb[NL];
+    double ts[NL], se[NL], sb[NL];
 #pragma unroll
     for (int L = 0; L < NL; ++L) {
-      if (doT) {
-        const double *row = rT + (long)L * rowstep;
-        tj0[L] = row[ix];
-        tj1[L] = row[ix + 1];
-        ts[L] = sk[(long)tpT[L] * tstride + so];
-      }
+      if (doT) ts[L] = sk[(long)W.tpT[L] * tstride + so];
       if (doP) {
-        const double *re = rE + (long)L * rowstep;
-        const double *rq = rM + (long)L * rowstep;
-        ej0[L] = re[ix];
-        ej1[L] = re[ix + 1];
-        qj0[L] = rq[ix];
-        qj1[L] = rq[ix + 1];
-        se[L] = sk[(long)tpE[L] * tstride + so];
-        sb[L] = sk[(long)tpB[L] * tstride + so];
+        se[L] = sk[(long)W.tpE[L] * tstride + so];
+        sb[L] = sk[(long)W.tpB[L] * tstride + so];
       }
     }
-    if (doT) {
-      double integrand = 0;
 #pragma unroll
-      for (int L = 0; L < NL; ++L)
-        if (ix >= xmT[L]) integrand += (aJ * tj0[L] + (1 - aJ) * tj1[L]) * ts[L];
-      accT += integrand * dt;
-    }
-    if (doP) {
-      double iEd = 0, iEm = 0, iBd = 0, iBm = 0;
+    for (int j = 0; j < NLB; ++j) {
+      if (!W.act[j]) continue;
+      const long lmo = (long)W.lm[j] * P.X + ix;
+      double tj0[NL], tj1[NL], ej0[NL], ej1[NL], qj0[NL], qj1[NL];
 #pragma unroll
       for (int L = 0; L < NL; ++L) {
-        if (ix >= xmE[L]) {
-          const double Jv = aJ * ej0[L] + (1 - aJ) * ej1[L];
-          if (P.has_E) iEd += Jv * se[L];
-          if (P.has_B) iBd += Jv * sb[L];
+        if (doT) {
+          const double *row = Jk + ((long)P.jTT * P.L_size + L) * rowstep + lmo;
+          tj0[L] = row[0];
+          tj1[L] = row[1];
         }
-        if (ix >= xmM[L]) {
-          const double Jv = aJ * qj0[L] + (1 - aJ) * qj1[L];
-          if (m != 0) iEm += Jv * sb[L];
-          iBm += Jv * se[L];
+        if (doP) {
+          const double *re = Jk + ((long)P.jEE * P.L_size + L) * rowstep + lmo;
+          const double *rq = Jk + ((long)jEB * P.L_size + L) * rowstep + lmo;
+          ej0[L] = re[0];
+          ej1[L] = re[1];
+          qj0[L] = rq[0];
+          qj1[L] = rq[1];
         }
       }
-      accEd += iEd * dt;
-      accEm += iEm * dt;
-      accBd += iBd * dt;
-      accBm += iBm * dt;
+      if (doT) {
+        double integrand = 0;
+#pragma unroll
+        for (int L = 0; L < NL; ++L)
+          if (ix >= W.xmT[j][L]) integrand += (aJ * tj0[L] + (1 - aJ) * tj1[L]) * ts[L];
+        accT[j] += integrand * dt;
+      }
+      if (doP) {
+        double iEd = 0, iEm = 0, iBd = 0, iBm = 0;
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          if (ix >= W.xmE[j][L]) {
+            const double Jv = aJ * ej0[L] + (1 - aJ) * ej1[L];
+            if (P.has_E) iEd += Jv * se[L];
+            if (P.has_B) iBd += Jv * sb[L];
+          }
+          if (ix >= W.xmM[j][L]) {
+            const double Jv = aJ * qj0[L] + (1 - aJ) * qj1[L];
+            if (m != 0) iEm += Jv * sb[L];
+            iBm += Jv * se[L];
+          }
+        }
+        accEd[j] += iEd * dt;
+        accEm[j] += iEm * dt;
+        accBd[j] += iBd * dt;
+        accBm[j] += iBm * dt;
+      }
     }
   }
-  if (!valid) return;
-  const double den = 2. * P.l_of[lm] + 1;
-  const long ocell = job.obase + P.ocol[sp];
-  if (P.has_T) {
-    double v = accT;
-    v *= 0.5;
-    v /= 4.;
-    v /= den;
-    v /= 2.;
-    P.out[ocell + (long)(job.ttT + lm) * job.ostride] = v;
-  }
-  if (P.has_E) {
-    double d = accEd;
-    d *= 0.5;
-    double mx = 0;
-    if (m != 0) {
-      mx = accEm;
+  if (!W.valid) return;
+  const long ocell = W.job.obase + P.ocol[W.sp];
+#pragma unroll
+  for (int j = 0; j < NLB; ++j) {
+    if (!W.act[j]) continue;
+    const int lm = W.lm[j];
+    const double den = 2. * P.l_of[lm] + 1;
+    if (P.has_T) {
+      double v = accT[j];
+      v *= 0.5;
+      v /= 4.;
+      v /= den;
+      v /= 2.;
+      P.out[ocell + (long)(W.job.ttT + lm) * W.job.ostride] = v;
+    }
+    if (P.has_E) {
+      double d = accEd[j];
+      d *= 0.5;
+      double mx = 0;
+      if (m != 0) {
+        mx = accEm[j];
+        mx *= 0.5;
+      }
+      double v = d + mx;
+      v /= 4.;
+      v /= den;
+      v /= 2.;
+      P.out[ocell + (long)(W.job.ttE + lm) * W.job.ostride] = v;
+    }
+    if (P.has_B) {
+      double d = accBd[j];
+      d *= 0.5;
+      double mx = accBm[j];
       mx *= 0.5;
+      double v = d - mx;
+      v /= 4.;
+      v /= den;
+      v /= 2.;
+      P.out[ocell + (long)(W.job.ttB + lm) * W.job.ostride] = v;
     }
-    double v = d + mx;
-    v /= 4.;
-    v /= den;
-    v /= 2.;
-    P.out[ocell + (long)(job.ttE + lm) * job.ostride] = v;
-  }
-  if (P.has_B) {
-    double d = accBd;
-    d *= 0.5;
-    double mx = accBm;
-    mx *= 0.5;
-    double v = d - mx;
-    v /= 4.;
-    v /= den;
-    v /= 2.;
-    P.out[ocell + (long)(job.ttB + lm) * job.ostride] = v;
   }
 }
