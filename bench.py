@@ -39,6 +39,7 @@ from song_amd import lib as L  # noqa: E402
 
 METRIC = ("LOS transfer elements/s (k1,k2,k3,l,m), FP32-comp vs FP64 at 1e-5 rel; p99 lat")
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = {"fp64": 78.6, "fp64fast": 78.6, "fp32c": 157.3}  # MI355X vector FP64 / FP32
 
 _C2 = dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=6, l_max_los_p=6, k_size=32,
            k_min=1e-5, k_max=0.1, k3_size=60, tau_size=600, tau_min=200.0, tau_max=14000.0,
@@ -301,6 +302,7 @@ def main(argv=None):
         eng.load(scen)
     print(f"# tables uploaded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     n_el = eng.n_elements()
+    flop_live, flop_dense = eng.flops()
     cols, sizes = loaded_columns(scen, pairs)
 
     def run_form(form):
@@ -332,6 +334,7 @@ def main(argv=None):
                "prep_ms": prep_ms, "graph": graph, "alg_bytes": eng.alg_bytes()}
         if los_ms > 0:
             res["hbm_gbs"] = res["alg_bytes"] / (los_ms * 1e-3) / 1e9
+            res["valu_tflops"] = flop_live / (los_ms * 1e-3) / 1e12
         kind, nk, ch, pers = eng.get_shape(FORMS[form])
         res["tile_shape"] = ({"kernel": "sweep", "nk": nk} if kind == 1 else
                              {"kernel": "k-lane", "l_per_lane": ch} if kind == 2 else
@@ -381,7 +384,11 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": KERNELS[args.form], "kernel_ms": head["kernel_ms"],
-                     "alg_bytes": alg_bytes},
+                     "alg_bytes": alg_bytes,
+                     "valu": {"achieved": head.get("valu_tflops", 0.0),
+                              "peak": VALU_PEAK_TFLOPS[args.form], "unit": "TFLOP/s",
+                              "frac": head.get("valu_tflops", 0.0) / VALU_PEAK_TFLOPS[args.form],
+                              "flop_alg": flop_live, "flop_dense": flop_dense}},
         "forms": results,
     }
     if not args.no_cpu:

@@ -105,6 +105,11 @@ int los_engine_store(los_engine *eng, struct transfers2 *ptr2, char *errmsg);
 
 /* Element count (tt x fine k3 summed over loaded pairs). */
 long los_engine_n_elements(los_engine *eng);
+/* Algorithmic operation count of one run of the loaded batch: per element and
+   integral, sum over tau of 5 per live L (J interpolation 3, multiply-add 2)
+   plus 2 when any L is live (trapezoid); `live` applies the reference's
+   index_xmin_J skip exactly, `dense` counts every L row. */
+int los_engine_flops(los_engine *eng, double *live, double *dense);
 /* Algorithmic bytes of one LOS kernel launch of the current form: the coarse
    source columns once (twice with the cubic k3 spline's second derivatives),
    the live part of the J table once (FP32 for the FP32-compensated form), the

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -101,6 +102,10 @@ static void free_tables(los_engine *e)
   dfree(e->dmi_of);
   dfree(e->dJp);
   dfree(e->dJp32);
+  for (int i = 0; i < 2; ++i) {
+    if (e->dJp2[i]) (void)hipFree(e->dJp2[i]);
+    e->dJp2[i] = nullptr;
+  }
   dfree(e->dJk);
   dfree(e->dJk32);
   for (int gs = 0; gs < 3; ++gs) dfree(e->dkgrp[gs]);
@@ -264,6 +269,8 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
     if (lm_of_lp[lp] >= 0) e->wave_minx[lp / 64] = std::min(e->wave_minx[lp / 64], hminx[lm_of_lp[lp]]);
 
   free_tables(e);
+  e->hxmin = hxmin;
+  e->hm_of = hm;
   e->n_lp = n_lp;
   e->m_size = m_size;
   e->n_wave = NLP / 64;
@@ -478,6 +485,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   for (int i = 0; i < e->F; ++i) forder[i] = i;
   std::stable_sort(forder.begin(), forder.end(), [&](int a, int b) { return kf[a] < kf[b]; });
   e->src_total = src_off;
+  e->ht0mt = t0mt;
   e->hforder = forder;
   e->hkf = kf;
   e->tt2_size = ptr2->tt2_size;
@@ -724,7 +732,7 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   }
   split_xcd(tiles, cost, P->xcd_tile);
   if (sh->kind == 1) {
-    P->Jp = fp32 ? (const void *)e->dJp32 : (const void *)e->dJp;
+    P->Jp = e->dJp2[fp32];
     HIPCHK(los_launch_sweep(fp32, P, sh->nk, s), err);
   } else {
     HIPCHK(los_launch_stream(fp32, P, sh, e->n_cu, s), err);
@@ -788,6 +796,15 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
   if (fp32 && e->dJp32 == nullptr) {
     HIPCHK(hipMalloc((void **)&e->dJp32, std::max<size_t>(1, e->nJp) * sizeof(float)), err);
     HIPCHK(los_launch_round_j32(e->dJp, e->dJp32, (long)e->nJp, e->stream), err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+  }
+  if (e->dJp2[fp32] == nullptr) {
+    const size_t rb = fp32 ? 4 : 8;
+    const int NLP = e->m_size * e->n_lp;
+    HIPCHK(hipMalloc(&e->dJp2[fp32], std::max<size_t>(1, e->nJp) * 2 * rb), err);
+    HIPCHK(los_launch_sweep_pairs(fp32, fp32 ? (const void *)e->dJp32 : (const void *)e->dJp,
+                                  e->dJp2[fp32], (long)e->J_size * e->L_size, e->X, NLP, e->stream),
+           err);
     HIPCHK(hipStreamSynchronize(e->stream), err);
   }
   if (e->sweep_fp32 == fp32) return _SUCCESS_;
@@ -1056,6 +1073,79 @@ extern "C" int los_engine_store(los_engine *e, struct transfers2 *ptr2, char *er
 }
 
 extern "C" long los_engine_n_elements(los_engine *e) { return (long)e->tt2_size * e->F; }
+
+// Algorithmic FP64/FP32 operation count of one run. For every output element
+// and each integral it sums (source/transfer2.c:2140-2341), over the time
+// steps, 5 per live L (3 for the J interpolation, 2 for the multiply-add with
+// the source) plus 2 when any L is live (the trapezoid multiply-add); `live`
+// applies the reference's skip rule exactly, `dense` counts every L row.
+extern "C" int los_engine_flops(los_engine *e, double *live, double *dense)
+{
+  if (!e->loaded) return _FAILURE_;
+  const los_params &P = e->P;
+  const int ntau = e->ntau, n_lm = e->n_lm, L_size = e->L_size;
+  const double xs = P.xx_step;
+  // the integrals of each (l,m): lists of J row indices (jt * L_size + L)
+  std::vector<std::vector<std::vector<int>>> calls(n_lm);
+  for (int lm = 0; lm < n_lm; ++lm) {
+    const int m = e->hm_of[lm];
+    auto rows = [&](int jt, int Lmax) {
+      std::vector<int> r;
+      for (int L = m; L <= Lmax && L < L_size; ++L) r.push_back(jt * L_size + L);
+      return r;
+    };
+    if (P.has_T && P.jTT >= 0) calls[lm].push_back(rows(P.jTT, P.LmaxT));
+    if (P.has_E && P.jEE >= 0) {
+      calls[lm].push_back(rows(P.jEE, P.LmaxP));
+      if (m != 0 && P.jEB >= 0 && P.has_B) calls[lm].push_back(rows(P.jEB, P.LmaxP));
+    }
+    if (P.has_B && P.jEE >= 0) {
+      calls[lm].push_back(rows(P.jEE, P.LmaxP));
+      if (P.jEB >= 0) calls[lm].push_back(rows(P.jEB, P.LmaxP));
+    }
+  }
+  double d = 0;
+  for (int lm = 0; lm < n_lm; ++lm)
+    for (const auto &c : calls[lm]) d += (double)ntau * (5.0 * c.size() + 2.0);
+  d *= e->F;
+  const unsigned nth = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<double> part(nth, 0.0);
+  std::vector<std::thread> th;
+  for (unsigned w = 0; w < nth; ++w)
+    th.emplace_back([&, w]() {
+      std::vector<int> ix(ntau);
+      double acc = 0;
+      for (int f = (int)w; f < e->F; f += (int)nth) {
+        const double k = e->hkf[f];
+        for (int t = 0; t < ntau; ++t) ix[t] = (int)(k * e->ht0mt[t] / xs);
+        auto steps = [&](int xm) {  // leading time steps with index_x >= xm
+          int lo = 0, hi = ntau;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ix[mid] >= xm) lo = mid + 1; else hi = mid;
+          }
+          return lo;
+        };
+        for (int lm = 0; lm < n_lm; ++lm)
+          for (const auto &c : calls[lm]) {
+            int most = 0;
+            for (int row : c) {
+              const int n = steps(e->hxmin[(size_t)row * n_lm + lm]);
+              acc += 5.0 * n;
+              most = std::max(most, n);
+            }
+            acc += 2.0 * most;
+          }
+      }
+      part[w] = acc;
+    });
+  for (auto &x : th) x.join();
+  double l = 0;
+  for (double v : part) l += v;
+  if (live) *live = l;
+  if (dense) *dense = d;
+  return _SUCCESS_;
+}
 
 extern "C" double los_engine_alg_bytes(los_engine *e)
 {

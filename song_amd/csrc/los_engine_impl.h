@@ -25,6 +25,8 @@ extern "C" hipError_t los_stream_set_attrs(int tp2_size);
 extern "C" int los_shapes(int fp32, int tp2_size, los_shape *out, int cap);
 extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, long rows, int X,
+                                             int NLP, hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_klane(int fp32, const los_params *P, int nlb, hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep_exact(const los_params *P, hipStream_t s);
@@ -48,11 +50,15 @@ struct los_engine {
   int *dl_of = nullptr, *dm_of = nullptr, *dmi_of = nullptr;
   int X = 0, L_size = 0, J_size = 0, n_lm = 0;
   int minx_all = INT_MAX;  // smallest first-live x index over all rows in use
+  std::vector<int> hxmin;  // host copy of xmin [jt][L][lm]
+  std::vector<int> hm_of;  // m of each (l,m) offset
+  std::vector<double> ht0mt;  // tau0 - tau of the loaded time grid
   // the same table laid out for the sweep kernels: [jt][L][x][lp], lane slot
   // lp = mi * n_lp + il (n_lp = l count rounded up to 64)
   double *dJp = nullptr;
   float *dJp32 = nullptr;
   size_t nJp = 0;
+  void *dJp2[2] = {nullptr, nullptr};  // (J[x], J[x+1]) pair tables, FP64 and FP32
   int *dxminp = nullptr, *dlm_of_lp = nullptr;
   int n_lp = 0, n_wave = 0, m_size = 0;
   std::vector<int> wave_minx;  // smallest first-live x index over the lanes of each wave class

@@ -67,6 +67,20 @@ hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s)
   return fp32 ? launch_sweep_prep<float>(P, s) : launch_sweep_prep<double>(P, s);
 }
 
+// Pair table of the sweep kernels from the sweep J table.
+hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, long rows, int X, int NLP,
+                                  hipStream_t s)
+{
+  if (rows * X * NLP == 0) return hipSuccess;
+  if (fp32)
+    hipLaunchKernelGGL(los_sweep_pairs<float>, dim3(4096), dim3(256), 0, s, (const float *)src,
+                       (SweepPair<float> *)dst, rows, X, NLP);
+  else
+    hipLaunchKernelGGL(los_sweep_pairs<double>, dim3(4096), dim3(256), 0, s, (const double *)src,
+                       (SweepPair<double> *)dst, rows, X, NLP);
+  return hipGetLastError();
+}
+
 hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s)
 {
   switch (nk) {

@@ -39,6 +39,30 @@ struct SweepS {
   R T, E, B, pad;
 };
 
+// Both interpolation nodes of a J row at one x slot, J[x] and J[x+1], side by
+// side so that a lane reads them with one load.
+template <class R>
+struct alignas(2 * sizeof(R)) SweepPair {
+  R j0, j1;
+};
+
+// Pair table [row][x][lp] from the sweep table [row][x][lp] (x + 1 = X reads 0).
+template <class R>
+__global__ void __launch_bounds__(256)
+los_sweep_pairs(const R *src, SweepPair<R> *dst, long rows, int X, int NLP)
+{
+  const long n = rows * X * NLP;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long lp = i % NLP;
+    const long rx = i / NLP;
+    const int x = (int)(rx % X);
+    SweepPair<R> p;
+    p.j0 = src[i];
+    p.j1 = x + 1 < X ? src[i + NLP] : (R)0;
+    dst[i] = p;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Per (fine k3, time step): index_x, a_J and S x dtau of every (m, L, field).
 // One thread per (f, t), t fastest (the tau-contiguous source layout).
@@ -140,7 +164,7 @@ los_sweep(los_params P)
       if (P.jEB >= 0) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
     }
   }
-  const R *Jb = (const R *)P.Jp + lp;
+  const SweepPair<R> *Jb = (const SweepPair<R> *)P.Jp + lp;
   const long rowstep = (long)P.X * NLP;
   const SweepIXA<R> *__restrict__ ixa = (const SweepIXA<R> *)P.ixa;
   const SweepS<R> *__restrict__ sf = (const SweepS<R> *)P.sf;
@@ -182,24 +206,24 @@ los_sweep(los_params P)
           }
           continue;
         }
-        const long o0 = (long)ix[j] * NLP, o1 = o0 + NLP;
+        const long o0 = (long)ix[j] * NLP;
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
           t0v[j][L] = t1v[j][L] = e0v[j][L] = e1v[j][L] = q0v[j][L] = q1v[j][L] = 0;
           if (m > L) continue;
           if (doT && L <= P.LmaxT) {
-            const R *row = Jb + (long)(P.jTT * P.L_size + L) * rowstep;
-            t0v[j][L] = row[o0];
-            t1v[j][L] = row[o1];
+            const SweepPair<R> w = Jb[(long)(P.jTT * P.L_size + L) * rowstep + o0];
+            t0v[j][L] = w.j0;
+            t1v[j][L] = w.j1;
           }
           if (doP && L <= P.LmaxP) {
-            const R *re = Jb + (long)(P.jEE * P.L_size + L) * rowstep;
-            e0v[j][L] = re[o0];
-            e1v[j][L] = re[o1];
+            const SweepPair<R> w = Jb[(long)(P.jEE * P.L_size + L) * rowstep + o0];
+            e0v[j][L] = w.j0;
+            e1v[j][L] = w.j1;
             if (mix) {
-              const R *rm = Jb + (long)(P.jEB * P.L_size + L) * rowstep;
-              q0v[j][L] = rm[o0];
-              q1v[j][L] = rm[o1];
+              const SweepPair<R> u = Jb[(long)(P.jEB * P.L_size + L) * rowstep + o0];
+              q0v[j][L] = u.j0;
+              q1v[j][L] = u.j1;
             }
           }
         }

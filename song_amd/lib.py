@@ -109,6 +109,7 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_last_kernel_ms": (i, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "los_engine_store": (i, [vp, vp, ch]),
         "los_engine_n_elements": (C.c_long, [vp]),
+        "los_engine_flops": (i, [vp, C.POINTER(d), C.POINTER(d)]),
         "los_engine_alg_bytes": (d, [vp]),
         "los_engine_stream": (vp, [vp]),
     }
@@ -423,6 +424,12 @@ class Engine:
 
     def alg_bytes(self) -> float:
         return self.lib.los_engine_alg_bytes(self.ptr)
+
+    def flops(self):
+        """(live, dense) algorithmic operation counts of one run."""
+        a, b = C.c_double(), C.c_double()
+        self.lib.los_engine_flops(self.ptr, C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def close(self) -> None:
         if self.ptr:
