@@ -732,7 +732,7 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   }
   split_xcd(tiles, cost, P->xcd_tile);
   if (sh->kind == 1) {
-    P->Jp = e->dJp2[fp32];
+    P->Jp = fp32 ? (const void *)e->dJp32 : e->dJp2[0];
     HIPCHK(los_launch_sweep(fp32, P, sh->nk, s), err);
   } else {
     HIPCHK(los_launch_stream(fp32, P, sh, e->n_cu, s), err);
@@ -798,12 +798,12 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
     HIPCHK(los_launch_round_j32(e->dJp, e->dJp32, (long)e->nJp, e->stream), err);
     HIPCHK(hipStreamSynchronize(e->stream), err);
   }
-  if (e->dJp2[fp32] == nullptr) {
-    const size_t rb = fp32 ? 4 : 8;
+  if (!fp32 && e->dJp2[0] == nullptr) {
+    // FP64: (J[x], J[x+1]) pairs, one load per row
     const int NLP = e->m_size * e->n_lp;
-    HIPCHK(hipMalloc(&e->dJp2[fp32], std::max<size_t>(1, e->nJp) * 2 * rb), err);
-    HIPCHK(los_launch_sweep_pairs(fp32, fp32 ? (const void *)e->dJp32 : (const void *)e->dJp,
-                                  e->dJp2[fp32], (long)e->J_size * e->L_size, e->X, NLP, e->stream),
+    HIPCHK(hipMalloc(&e->dJp2[0], std::max<size_t>(1, e->nJp) * 2 * sizeof(double)), err);
+    HIPCHK(los_launch_sweep_pairs(0, e->dJp, e->dJp2[0], (long)e->J_size * e->L_size, e->X, NLP,
+                                  e->stream),
            err);
     HIPCHK(hipStreamSynchronize(e->stream), err);
   }

@@ -46,6 +46,28 @@ struct alignas(2 * sizeof(R)) SweepPair {
   R j0, j1;
 };
 
+// J[x] and J[x+1] of a row at lane slot lp. FP64: from the pair table (one
+// load); FP32: from the plain sweep table (two loads, which share cache lines
+// with the neighbouring time steps).
+template <class R>
+__device__ __forceinline__ void sweep_row(const void *Jb, long rowoff, long o0, int NLP, R &j0, R &j1);
+template <>
+__device__ __forceinline__ void sweep_row<double>(const void *Jb, long rowoff, long o0, int NLP,
+                                                 double &j0, double &j1)
+{
+  const SweepPair<double> w = ((const SweepPair<double> *)Jb)[rowoff + o0];
+  j0 = w.j0;
+  j1 = w.j1;
+}
+template <>
+__device__ __forceinline__ void sweep_row<float>(const void *Jb, long rowoff, long o0, int NLP,
+                                                float &j0, float &j1)
+{
+  const float *r = (const float *)Jb + rowoff + o0;
+  j0 = r[0];
+  j1 = r[NLP];
+}
+
 // Pair table [row][x][lp] from the sweep table [row][x][lp] (x + 1 = X reads 0).
 template <class R>
 __global__ void __launch_bounds__(256)
@@ -164,7 +186,9 @@ los_sweep(los_params P)
       if (P.jEB >= 0) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
     }
   }
-  const SweepPair<R> *Jb = (const SweepPair<R> *)P.Jp + lp;
+  // FP64: pair table, FP32: plain table (both [row][x][lp]), offset to this lane
+  const void *Jb = sizeof(R) == 8 ? (const void *)((const SweepPair<double> *)P.Jp + lp)
+                                  : (const void *)((const float *)P.Jp + lp);
   const long rowstep = (long)P.X * NLP;
   const SweepIXA<R> *__restrict__ ixa = (const SweepIXA<R> *)P.ixa;
   const SweepS<R> *__restrict__ sf = (const SweepS<R> *)P.sf;
@@ -189,60 +213,38 @@ los_sweep(los_params P)
         ix[j] = v.ix;
         a[j] = v.a;
       }
-      // all J loads of this time step first (elements whose index_x equals the
-      // previous element's reuse its values), then the arithmetic
-      R t0v[NK][NL], t1v[NK][NL], e0v[NK][NL], e1v[NK][NL], q0v[NK][NL], q1v[NK][NL];
+      // the NK elements in turn; the J rows are loaded again only when an
+      // element's index_x differs from the previous one's (neighbouring k3
+      // usually share it), so one set of row registers serves the tile
+      R t0v[NL], t1v[NL], e0v[NL], e1v[NL], q0v[NL], q1v[NL];
 #pragma unroll
       for (int j = 0; j < NK; ++j) {
-        if (j > 0 && ix[j] == ix[j - 1]) {
+        if (j == 0 || ix[j] != ix[j - 1]) {
+          const long o0 = (long)ix[j] * NLP;
 #pragma unroll
           for (int L = 0; L < NL; ++L) {
-            t0v[j][L] = t0v[j - 1][L];
-            t1v[j][L] = t1v[j - 1][L];
-            e0v[j][L] = e0v[j - 1][L];
-            e1v[j][L] = e1v[j - 1][L];
-            q0v[j][L] = q0v[j - 1][L];
-            q1v[j][L] = q1v[j - 1][L];
-          }
-          continue;
-        }
-        const long o0 = (long)ix[j] * NLP;
-#pragma unroll
-        for (int L = 0; L < NL; ++L) {
-          t0v[j][L] = t1v[j][L] = e0v[j][L] = e1v[j][L] = q0v[j][L] = q1v[j][L] = 0;
-          if (m > L) continue;
-          if (doT && L <= P.LmaxT) {
-            const SweepPair<R> w = Jb[(long)(P.jTT * P.L_size + L) * rowstep + o0];
-            t0v[j][L] = w.j0;
-            t1v[j][L] = w.j1;
-          }
-          if (doP && L <= P.LmaxP) {
-            const SweepPair<R> w = Jb[(long)(P.jEE * P.L_size + L) * rowstep + o0];
-            e0v[j][L] = w.j0;
-            e1v[j][L] = w.j1;
-            if (mix) {
-              const SweepPair<R> u = Jb[(long)(P.jEB * P.L_size + L) * rowstep + o0];
-              q0v[j][L] = u.j0;
-              q1v[j][L] = u.j1;
+            t0v[L] = t1v[L] = e0v[L] = e1v[L] = q0v[L] = q1v[L] = 0;
+            if (m > L) continue;
+            if (doT && L <= P.LmaxT) sweep_row(Jb, (long)(P.jTT * P.L_size + L) * rowstep, o0, NLP, t0v[L], t1v[L]);
+            if (doP && L <= P.LmaxP) {
+              sweep_row(Jb, (long)(P.jEE * P.L_size + L) * rowstep, o0, NLP, e0v[L], e1v[L]);
+              if (mix) sweep_row(Jb, (long)(P.jEB * P.L_size + L) * rowstep, o0, NLP, q0v[L], q1v[L]);
             }
           }
         }
-      }
-#pragma unroll
-      for (int j = 0; j < NK; ++j) {
         const SweepS<R> *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * NL;
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
           if (m > L) continue;
           const SweepS<R> s = sp[L];
           if (doT && L <= P.LmaxT) {
-            R Jv = t1v[j][L] + a[j] * (t0v[j][L] - t1v[j][L]);
+            R Jv = t1v[L] + a[j] * (t0v[L] - t1v[L]);
             Jv = ix[j] >= xmT[L] ? Jv : (R)0;
             pT[j] += Jv * s.T;
           }
           if (doP && L <= P.LmaxP) {
-            R Je = e1v[j][L] + a[j] * (e0v[j][L] - e1v[j][L]);
-            R Jm = q1v[j][L] + a[j] * (q0v[j][L] - q1v[j][L]);
+            R Je = e1v[L] + a[j] * (e0v[L] - e1v[L]);
+            R Jm = q1v[L] + a[j] * (q0v[L] - q1v[L]);
             Je = ix[j] >= xmE[L] ? Je : (R)0;
             Jm = ix[j] >= xmM[L] ? Jm : (R)0;
             pE[j] += Je * s.E + Jm * (emix * s.B);
