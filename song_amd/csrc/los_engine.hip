@@ -64,7 +64,7 @@ static void free_batch(los_engine *e)
   if (e->dixa) (void)hipFree(e->dixa);
   if (e->dsf) (void)hipFree(e->dsf);
   e->dixa = e->dsf = nullptr;
-  e->sweep_fp32 = -1;
+  e->sweep_form = -1;
   e->sweep_bytes = 0;
   dfree(e->dpairs);
   dfree(e->dkc);
@@ -699,6 +699,7 @@ static int form_shapes(int form, int tp2_size, los_shape *out, int cap)
   if (form == LOS_FORM_FP64) {
     int n = 0;
     if (n < cap) out[n++] = los_shape{0, 1, 0, 0};
+    for (int nk = 1; nk <= 4 && n < cap; nk *= 2) out[n++] = los_shape{1, nk, 0, 0};
     for (int b = 1; b <= 4 && n < cap; b *= 2) out[n++] = los_shape{2, 64, b, 0};
     return n;
   }
@@ -725,6 +726,12 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
       HIPCHK(los_launch_klane(fp32, P, sh->ch, s), err);
     return _SUCCESS_;
   }
+  if (e->form == LOS_FORM_FP64 && sh->kind == 1) {
+    P->Jp = e->dJp2[0];
+    split_xcd(tiles, cost, P->xcd_tile);
+    HIPCHK(los_launch_sweep_exact(P, sh->nk, s), err);
+    return _SUCCESS_;
+  }
   if (e->form == LOS_FORM_FP64) {
     split_xcd(tiles, cost, P->xcd_pos);
     HIPCHK(los_launch_integrate_exact(P, 0, s), err);
@@ -745,12 +752,15 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
 static int enqueue_tables(los_engine *e, const los_shape *sh, char *err)
 {
   const int fp32 = e->form == LOS_FORM_FP32C;
-  if (sh->kind == 1 && e->form != LOS_FORM_FP64) {
-    if (e->sweep_fp32 != fp32) {
+  if (sh->kind == 1) {
+    if (e->sweep_form != e->form) {
       snprintf(err, _ERRORMSGSIZE_, "enqueue_tables: sweep tables not allocated for this form");
       return _FAILURE_;
     }
-    HIPCHK(los_launch_sweep_prep(fp32, &e->P, e->stream), err);
+    if (e->form == LOS_FORM_FP64)
+      HIPCHK(los_launch_sweep_prep_exact(&e->P, e->stream), err);
+    else
+      HIPCHK(los_launch_sweep_prep(fp32, &e->P, e->stream), err);
   } else if (sh->kind == 2) {
     if (e->klane_form != e->form) {
       snprintf(err, _ERRORMSGSIZE_, "enqueue_tables: k-lane tables not allocated for this form");
@@ -791,7 +801,7 @@ int make_sweep_tables(los_engine *e, char *err)
 // Device tables a sweep shape needs (allocated outside any capture).
 int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
 {
-  if (sh == nullptr || sh->kind != 1 || e->form == LOS_FORM_FP64) return _SUCCESS_;
+  if (sh == nullptr || sh->kind != 1) return _SUCCESS_;
   const int fp32 = e->form == LOS_FORM_FP32C;
   if (fp32 && e->dJp32 == nullptr) {
     HIPCHK(hipMalloc((void **)&e->dJp32, std::max<size_t>(1, e->nJp) * sizeof(float)), err);
@@ -807,11 +817,11 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
            err);
     HIPCHK(hipStreamSynchronize(e->stream), err);
   }
-  if (e->sweep_fp32 == fp32) return _SUCCESS_;
+  if (e->sweep_form == e->form) return _SUCCESS_;
   if (e->dixa) (void)hipFree(e->dixa);
   if (e->dsf) (void)hipFree(e->dsf);
   e->dixa = e->dsf = nullptr;
-  e->sweep_fp32 = -1;
+  e->sweep_form = -1;
   const int NL = std::max(e->P.LmaxT, e->P.LmaxP) + 1;
   const size_t n = (size_t)e->F * e->ntau;
   const size_t rb = fp32 ? 4 : 8;
@@ -821,7 +831,7 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
   HIPCHK(hipMalloc(&e->dsf, sf_b), err);
   e->P.ixa = e->dixa;
   e->P.sf = e->dsf;
-  e->sweep_fp32 = fp32;
+  e->sweep_form = e->form;
   e->sweep_bytes = ixa_b + sf_b;
   return _SUCCESS_;
 }

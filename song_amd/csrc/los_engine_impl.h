@@ -25,6 +25,8 @@ extern "C" hipError_t los_stream_set_attrs(int tp2_size);
 extern "C" int los_shapes(int fp32, int tp2_size, los_shape *out, int cap);
 extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_prep_exact(const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_exact(const los_params *P, int nk, hipStream_t s);
 extern "C" hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, long rows, int X,
                                              int NLP, hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s);
@@ -96,7 +98,7 @@ struct los_engine {
 
   int *dtau_end_fw = nullptr;          // per (fine k3, wave class)
   void *dixa = nullptr, *dsf = nullptr;  // per-(k3, tau) tables of the sweep kernels
-  int sweep_fp32 = -1;                   // precision they hold (-1: none)
+  int sweep_form = -1;                   // form whose tables they hold (-1: none)
   size_t sweep_bytes = 0;
   int *drank = nullptr;                  // rank (sorted position) of each fine index
   void *dkixa = nullptr, *dksrc = nullptr;  // tables of the k-lane kernels

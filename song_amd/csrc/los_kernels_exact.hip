@@ -18,6 +18,7 @@
 
 #include "los_device.h"
 #include "los_klane.h"
+#include "los_sweep.h"
 
 extern "C" {
 
@@ -381,6 +382,22 @@ hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double
   hipLaunchKernelGGL(los_spline_k3_tau, dim3((unsigned)grid), dim3(block), 0, s, pairs, ncols, kc,
                      spl, src, dd, ntau, tp2);
   return hipGetLastError();
+}
+
+// sweep organisation of the plain FP64 form (same per-element operations)
+hipError_t los_launch_sweep_prep_exact(const los_params *P, hipStream_t s)
+{
+  return launch_sweep_prep<double, false>(P, s);
+}
+
+hipError_t los_launch_sweep_exact(const los_params *P, int nk, hipStream_t s)
+{
+  switch (nk) {
+    case 1: return launch_sweep_exact<1>(P, s);
+    case 2: return launch_sweep_exact<2>(P, s);
+    case 4: return launch_sweep_exact<4>(P, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // k-lane organisation of the plain FP64 form (same per-element operations).

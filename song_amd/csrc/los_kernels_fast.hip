@@ -64,7 +64,7 @@ hipError_t los_stream_set_attrs(int tp2_size)
 // nk positions (1, 2 or 4).
 hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s)
 {
-  return fp32 ? launch_sweep_prep<float>(P, s) : launch_sweep_prep<double>(P, s);
+  return fp32 ? launch_sweep_prep<float, true>(P, s) : launch_sweep_prep<double, true>(P, s);
 }
 
 // Pair table of the sweep kernels from the sweep J table.

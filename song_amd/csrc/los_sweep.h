@@ -89,7 +89,7 @@ los_sweep_pairs(const R *src, SweepPair<R> *dst, long rows, int X, int NLP)
 // Per (fine k3, time step): index_x, a_J and S x dtau of every (m, L, field).
 // One thread per (f, t), t fastest (the tau-contiguous source layout).
 // ---------------------------------------------------------------------------
-template <class R>
+template <class R, bool WITH_DT>
 __global__ void __launch_bounds__(256)
 los_sweep_prep(los_params P, int NL)
 {
@@ -118,12 +118,14 @@ los_sweep_prep(los_params P, int NL)
     if (tp < 0) return 0.0;
     const long base = pr.src_off + ((long)tp * pr.nc + br) * P.ntau + t;
     const double *S = P.src + base;
-    double v = aw * S[0] + bw * S[P.ntau];
+    double v;
     if (P.cubic_k3) {
       const double *D = P.dd + base;
-      v += (ca * D[0] + cb * D[P.ntau]) * h * h / 6.0;
+      v = aw * S[0] + bw * S[P.ntau] + (ca * D[0] + cb * D[P.ntau]) * h * h / 6.0;
+    } else {
+      v = aw * S[0] + bw * S[P.ntau];
     }
-    return v * dt;
+    return WITH_DT ? v * dt : v;
   };
   SweepS<R> *out = (SweepS<R> *)P.sf + (id * P.m_size) * NL;
   for (int mi = 0; mi < P.m_size; ++mi)
@@ -272,13 +274,14 @@ los_sweep(los_params P)
   }
 }
 
-template <class R>
+template <class R, bool WITH_DT>
 static hipError_t launch_sweep_prep(const los_params *P, hipStream_t s)
 {
   const long n = (long)P->F * P->ntau;
   if (n == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  hipLaunchKernelGGL(los_sweep_prep<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, *P, NL);
+  hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     s, *P, NL);
   return hipGetLastError();
 }
 
@@ -300,6 +303,187 @@ static hipError_t launch_sweep(const los_params *P, hipStream_t s)
   case n: hipLaunchKernelGGL((los_sweep<Pol, NK, n>), grid, dim3(64), 0, s, *P); break;
     LOS_SW(1) LOS_SW(2) LOS_SW(3) LOS_SW(4) LOS_SW(5) LOS_SW(6) LOS_SW(7) LOS_SW(8)
 #undef LOS_SW
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP64 form in the sweep organisation (instantiate without
+// floating-point contraction): the tables hold the reference's interpolated
+// sources (no trapezoid weight), and each element follows the reference's
+// operations: per time step the integrand of each integral is the sum over
+// live L, in L order, of (a_J J_i + (1 - a_J) J_{i+1}) S; the integral adds
+// integrand x dtau; E adds the B->E mixing integral only for m != 0, B always
+// subtracts the E->B one; then x 1/2, / 4, / (2l+1), / 2
+// (source/transfer2.c:2259-2329, :1861-2098).
+// ---------------------------------------------------------------------------
+template <int NK, int NL>
+__global__ void __launch_bounds__(64)
+los_sweep_exact(los_params P)
+{
+  const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
+  if (q >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return;
+  const int tile = P.xcd_tile[xcd + 1] - 1 - q;
+  const int w = blockIdx.y;
+  const int nchunk = P.n_lp >> 6;
+  const int mi = w / nchunk;
+  const int m = P.m_of_mi[mi];
+  const int lp = mi * P.n_lp + (w - mi * nchunk) * 64 + threadIdx.x;
+  const los_tile tl = P.tiles[tile];
+  const los_job job = P.jobs[P.pos_job[tl.pos0]];
+  const int lm = P.lm_of_lp[lp];
+  const bool active = lm >= 0 && lm < job.n_lm;
+  if (__ballot(active) == 0) return;
+
+  const int NLP = P.m_size * P.n_lp;
+  int f[NK];
+  int tend = 0;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    f[j] = P.forder[tl.pos0 + min(j, tl.npos - 1)];
+    tend = max(tend, P.tau_end_fw[(long)f[j] * P.n_wave + w]);
+  }
+  // rows the reference reads for this (l,m) (INT_MAX: skipped at every tau)
+  int xmT[NL], xmE[NL], xmM[NL];
+#pragma unroll
+  for (int L = 0; L < NL; ++L) {
+    xmT[L] = xmE[L] = xmM[L] = INT_MAX;
+    if (!active || m > L || L >= P.L_size) continue;
+    if (P.has_T && L <= P.LmaxT && P.jTT >= 0)
+      xmT[L] = P.xminp[((long)P.jTT * P.L_size + L) * NLP + lp];
+    if ((P.has_E || P.has_B) && L <= P.LmaxP) {
+      if (P.jEE >= 0) xmE[L] = P.xminp[((long)P.jEE * P.L_size + L) * NLP + lp];
+      if (P.jEB >= 0 && P.has_B) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
+    }
+  }
+  const void *Jb = (const void *)((const SweepPair<double> *)P.Jp + lp);
+  const long rowstep = (long)P.X * NLP;
+  const SweepIXA<double> *__restrict__ ixa = (const SweepIXA<double> *)P.ixa;
+  const SweepS<double> *__restrict__ sf = (const SweepS<double> *)P.sf;
+  const bool doT = P.has_T && P.jTT >= 0, doP = (P.has_E || P.has_B) && P.jEE >= 0;
+  const int jEB = P.jEB >= 0 ? P.jEB : P.jEE;
+
+  double accT[NK], accEd[NK], accEm[NK], accBd[NK], accBm[NK];
+#pragma unroll
+  for (int j = 0; j < NK; ++j) accT[j] = accEd[j] = accEm[j] = accBd[j] = accBm[j] = 0;
+  for (int t = 0; t < tend; ++t) {
+    const double dt = P.dtau[t];
+    int ix[NK];
+    double a[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+      const SweepIXA<double> v = ixa[(long)f[j] * P.ntau + t];
+      ix[j] = v.ix;
+      a[j] = v.a;
+    }
+    double t0v[NL], t1v[NL], e0v[NL], e1v[NL], q0v[NL], q1v[NL];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+      if (j == 0 || ix[j] != ix[j - 1]) {
+        const long o0 = (long)ix[j] * NLP;
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          t0v[L] = t1v[L] = e0v[L] = e1v[L] = q0v[L] = q1v[L] = 0;
+          if (m > L) continue;
+          if (doT && L <= P.LmaxT) sweep_row(Jb, (long)(P.jTT * P.L_size + L) * rowstep, o0, NLP, t0v[L], t1v[L]);
+          if (doP && L <= P.LmaxP) {
+            sweep_row(Jb, (long)(P.jEE * P.L_size + L) * rowstep, o0, NLP, e0v[L], e1v[L]);
+            sweep_row(Jb, (long)(jEB * P.L_size + L) * rowstep, o0, NLP, q0v[L], q1v[L]);
+          }
+        }
+      }
+      const SweepS<double> *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * NL;
+      const double aJ = a[j];
+      if (doT) {
+        double integrand = 0;
+#pragma unroll
+        for (int L = 0; L < NL; ++L)
+          if (ix[j] >= xmT[L]) integrand += (aJ * t0v[L] + (1 - aJ) * t1v[L]) * sp[L].T;
+        accT[j] += integrand * dt;
+      }
+      if (doP) {
+        double iEd = 0, iEm = 0, iBd = 0, iBm = 0;
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          const SweepS<double> s = sp[L];
+          if (ix[j] >= xmE[L]) {
+            const double Jv = aJ * e0v[L] + (1 - aJ) * e1v[L];
+            if (P.has_E) iEd += Jv * s.E;
+            if (P.has_B) iBd += Jv * s.B;
+          }
+          if (ix[j] >= xmM[L]) {
+            const double Jv = aJ * q0v[L] + (1 - aJ) * q1v[L];
+            if (m != 0) iEm += Jv * s.B;
+            iBm += Jv * s.E;
+          }
+        }
+        accEd[j] += iEd * dt;
+        accEm[j] += iEm * dt;
+        accBd[j] += iBd * dt;
+        accBm[j] += iBm * dt;
+      }
+    }
+  }
+  if (!active) return;
+  const double den = 2. * P.l_of[lm] + 1;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    if (j >= tl.npos) continue;
+    const long o = job.obase + P.ocol[tl.pos0 + j];
+    if (P.has_T) {
+      double v = accT[j];
+      v *= 0.5;
+      v /= 4.;
+      v /= den;
+      v /= 2.;
+      P.out[o + (long)(job.ttT + lm) * job.ostride] = v;
+    }
+    if (P.has_E) {
+      double d = accEd[j];
+      d *= 0.5;
+      double mx = 0;
+      if (m != 0) {
+        mx = accEm[j];
+        mx *= 0.5;
+      }
+      double v = d + mx;
+      v /= 4.;
+      v /= den;
+      v /= 2.;
+      P.out[o + (long)(job.ttE + lm) * job.ostride] = v;
+    }
+    if (P.has_B) {
+      double d = accBd[j];
+      d *= 0.5;
+      double mx = accBm[j];
+      mx *= 0.5;
+      double v = d - mx;
+      v /= 4.;
+      v /= den;
+      v /= 2.;
+      P.out[o + (long)(job.ttB + lm) * job.ostride] = v;
+    }
+  }
+}
+
+template <int NK>
+static hipError_t launch_sweep_exact(const los_params *P, hipStream_t s)
+{
+  if (P->n_pos == 0 || P->n_wave == 0) return hipSuccess;
+  int most = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int n = P->xcd_tile[x + 1] - P->xcd_tile[x];
+    most = n > most ? n : most;
+  }
+  if (most == 0) return hipSuccess;
+  const dim3 grid(8 * most, P->n_wave);
+  const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
+  switch (NL) {
+#define LOS_SWX(n) \
+  case n: hipLaunchKernelGGL((los_sweep_exact<NK, n>), grid, dim3(64), 0, s, *P); break;
+    LOS_SWX(1) LOS_SWX(2) LOS_SWX(3) LOS_SWX(4) LOS_SWX(5) LOS_SWX(6) LOS_SWX(7) LOS_SWX(8)
+#undef LOS_SWX
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

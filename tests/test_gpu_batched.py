@@ -44,7 +44,8 @@ def test_every_tile_shape(oracle):
     eng = L.Engine(0)
     eng.load(s)
     eng.set_form(L.FORM_FP64)
-    for sh in ((0, 1, 0), (2, 64, 1), (2, 64, 2), (2, 64, 4)):  # every plain kernel is bit-exact
+    for sh in ((0, 1, 0), (1, 1, 0), (1, 2, 0), (1, 4, 0), (2, 64, 1), (2, 64, 2), (2, 64, 4)):
+        # every kernel of the plain form is bit-exact
         eng.set_shape(L.FORM_FP64, *sh)
         print(f"form 0 shape {sh}", flush=True)
         assert np.array_equal(_run(eng, s, L.FORM_FP64), ref), f"fp64 shape {sh}"
@@ -98,7 +99,7 @@ def test_uneven_k3_counts(oracle):
     eng = L.Engine(0)
     eng.load(s)
     eng.set_form(L.FORM_FP64)
-    for sh in ((0, 1, 0), (2, 64, 2)):
+    for sh in ((0, 1, 0), (1, 2, 0), (2, 64, 2)):
         eng.set_shape(L.FORM_FP64, *sh)
         assert np.array_equal(_run(eng, s, L.FORM_FP64), ref), f"fp64 shape {sh}"
     tt2 = _tt2(s, ref)
@@ -118,6 +119,7 @@ def test_uneven_k3_counts(oracle):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form,shape", [(L.FORM_FP64, (0, 1)), (L.FORM_FP64, (2, 64, 2)),
+                                        (L.FORM_FP64, (1, 2)),
                                         (L.FORM_FP32C, (0, 4, 32, 1)),
                                         (L.FORM_FP64_FAST, (0, 8, 16, 0)),
                                         (L.FORM_FP32C, (1, 2, 0, 0)),
