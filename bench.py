@@ -273,6 +273,8 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--errors-out", default="", help="write per-(l,m) errors as JSON here")
+    ap.add_argument("--copy-probe-mib", type=float, default=0.0,
+                    help="run a plain device copy of this size once (HBM counter calibration)")
     ap.add_argument("--serve-budget", type=float, default=8.0,
                     help="c5: HBM budget of the admitted jobs, GiB")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
@@ -303,6 +305,13 @@ def main(argv=None):
     print(f"# tables uploaded in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     n_el = eng.n_elements()
     flop_live, flop_dense = eng.flops()
+    probe = None
+    if args.copy_probe_mib > 0:
+        nb = args.copy_probe_mib * 2.0 ** 20
+        pms = eng.copy_probe(nb)
+        probe = {"bytes_read": nb, "bytes_written": nb, "ms": pms,
+                 "gbs": 2 * nb / (pms * 1e-3) / 1e9 if pms > 0 else None}
+        print(f"# copy probe {args.copy_probe_mib} MiB: {pms:.3f} ms", file=sys.stderr, flush=True)
     cols, sizes = loaded_columns(scen, pairs)
 
     def run_form(form):
@@ -391,6 +400,8 @@ def main(argv=None):
                               "flop_alg": flop_live, "flop_dense": flop_dense}},
         "forms": results,
     }
+    if probe:
+        line["copy_probe"] = probe
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, pairs, args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -110,6 +110,9 @@ long los_engine_n_elements(los_engine *eng);
    plus 2 when any L is live (trapezoid); `live` applies the reference's
    index_xmin_J skip exactly, `dense` counts every L row. */
 int los_engine_flops(los_engine *eng, double *live, double *dense);
+/* A plain device copy of `bytes` (two launches of los_copy_probe, the second
+   timed): the kernel of known traffic that HBM counter readings are scaled by. */
+int los_engine_copy_probe(los_engine *eng, double bytes, float *ms, char *errmsg);
 /* Algorithmic bytes of one LOS kernel launch of the current form: the coarse
    source columns once (twice with the cubic k3 spline's second derivatives),
    the live part of the J table once (FP32 for the FP32-compensated form), the

@@ -1084,6 +1084,31 @@ extern "C" int los_engine_store(los_engine *e, struct transfers2 *ptr2, char *er
 
 extern "C" long los_engine_n_elements(los_engine *e) { return (long)e->tt2_size * e->F; }
 
+extern "C" hipError_t los_launch_copy_probe(const double *src, double *dst, long n, hipStream_t s);
+
+// One launch of a plain copy kernel of `bytes` on the engine's stream (device
+// to device), timed with HIP events: the reference of known traffic for HBM
+// counter readings of the LOS kernels.
+extern "C" int los_engine_copy_probe(los_engine *e, double bytes, float *ms, char *err)
+{
+  HIPCHK(hipSetDevice(e->device), err);
+  const long n = (long)(bytes / 8);
+  double *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc((void **)&a, std::max<long>(1, n) * 8), err);
+  HIPCHK(hipMalloc((void **)&b, std::max<long>(1, n) * 8), err);
+  HIPCHK(los_launch_copy_probe(b, a, n, e->stream), err);  // first touch
+  HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
+  HIPCHK(los_launch_copy_probe(a, b, n, e->stream), err);
+  HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
+  HIPCHK(hipEventSynchronize(e->ev[1]), err);
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e->ev[0], e->ev[1]), err);
+  if (ms) *ms = t;
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return _SUCCESS_;
+}
+
 // Algorithmic FP64/FP32 operation count of one run. For every output element
 // and each integral it sums (source/transfer2.c:2140-2341), over the time
 // steps, 5 per live L (3 for the J interpolation, 2 for the multiply-add with

@@ -38,7 +38,22 @@ static hipError_t set_attrs(int tp2_size)
   return e;
 }
 
+// Plain copy of n doubles (the reference kernel of known traffic that HBM
+// counters are calibrated against).
+__global__ void __launch_bounds__(256) los_copy_probe(const double *src, double *dst, long n)
+{
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 extern "C" {
+
+hipError_t los_launch_copy_probe(const double *src, double *dst, long n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(los_copy_probe, dim3(8192), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
 
 hipError_t los_launch_round_j32(const double *src, float *dst, long n, hipStream_t s)
 {

@@ -110,6 +110,7 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_store": (i, [vp, vp, ch]),
         "los_engine_n_elements": (C.c_long, [vp]),
         "los_engine_flops": (i, [vp, C.POINTER(d), C.POINTER(d)]),
+        "los_engine_copy_probe": (i, [vp, d, C.POINTER(C.c_float), ch]),
         "los_engine_alg_bytes": (d, [vp]),
         "los_engine_stream": (vp, [vp]),
     }
@@ -424,6 +425,12 @@ class Engine:
 
     def alg_bytes(self) -> float:
         return self.lib.los_engine_alg_bytes(self.ptr)
+
+    def copy_probe(self, nbytes: float) -> float:
+        """ms of one plain device copy of nbytes."""
+        ms = C.c_float()
+        self._call("los_engine_copy_probe", float(nbytes), C.byref(ms))
+        return ms.value
 
     def flops(self):
         """(live, dense) algorithmic operation counts of one run."""

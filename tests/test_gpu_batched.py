@@ -143,3 +143,45 @@ def test_graph_replay_matches_eager(form, shape):
         eng.set_form(form)
     eng.close()
     s.close()
+
+
+@pytest.mark.gpu
+def test_flop_count_matches_brute_force():
+    """los_engine_flops: sum over elements and integrals of sum_tau (5 L_live + 2 [L_live > 0])
+    with the reference's skip rule, recounted here from the J rows' first live indices."""
+    import ctypes as C
+
+    cfg = L.Config(**CASES["TEB_m012"])
+    s = L.Scenario(cfg)
+    s.prepare()
+    eng = L.Engine(0)
+    eng.load(s)
+    live, dense = eng.flops()
+    ls, _, ts = s.grids()
+    t0mt = cfg.tau0 - ts
+    lib = L.load()
+    kf = []
+    for (i1, i2), (_, n) in s.pair_offsets().items():
+        buf, last = (C.c_double * n)(), C.c_int()
+        lib.transfer2_get_k3_list(*[s.s[x] for x in ("ppr", "ppr2", "ppt2", "pbs", "pbs2", "ptr2")],
+                                  i1, i2, buf, C.byref(last))
+        kf.extend(list(buf))
+    tot = dtot = 0
+    for k in kf:
+        ix = (k * t0mt / cfg.xx_step).astype(np.int64)
+        for il, l in enumerate(ls):
+            for im, m in enumerate(cfg.m_list):
+                if m > l:
+                    continue
+                calls = [(0, cfg.l_max_los_t), (1, cfg.l_max_los_p)]  # T: TT; E: EE
+                if m != 0:
+                    calls.append((2, cfg.l_max_los_p))                # E: EB x S_B
+                calls += [(1, cfg.l_max_los_p), (2, cfg.l_max_los_p)]  # B: EE, EB
+                for jt, lmax in calls:
+                    xms = [s.J_row(jt, Lv, il, im)[0] for Lv in range(m, lmax + 1)]
+                    n_live = sum((ix >= x).astype(np.int64) for x in xms)
+                    tot += 5 * int(n_live.sum()) + 2 * int((n_live > 0).sum())
+                    dtot += len(ts) * (5 * len(xms) + 2)
+    eng.close()
+    s.close()
+    assert live == tot and dense == dtot
