@@ -70,8 +70,35 @@ DESCR = {
 }
 
 FORMS = {"fp64": L.FORM_FP64, "fp64fast": L.FORM_FP64_FAST, "fp32c": L.FORM_FP32C}
-KERNELS = {"fp64": "los_integrate_exact", "fp64fast": "<StreamFP64> (stream or sweep, tuned)",
-           "fp32c": "<StreamFP32C> (stream or sweep, tuned)"}
+POLICY = {"fp64fast": "StreamFP64", "fp32c": "StreamFP32C"}
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a copy
+# kernel of known size), keyed by kernel name; the bench reports them for the kernel it ran
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "los_{workload}_hbm_traffic.json")
+
+
+def kernel_name(form: str, shape, nl: int) -> str:
+    """The LOS kernel a tuned shape launches, named as rocprofv3 lists it."""
+    kind, nk, ch, pers = shape
+    if form == "fp64":
+        return (f"los_sweep_exact<{nk}, {nl}>" if kind == 1 else
+                f"los_klane_exact<{nl}, {ch}>" if kind == 2 else f"los_integrate_exact<{nl}>")
+    pol = POLICY[form]
+    if kind == 1:
+        return f"los_sweep<{pol}, {nk}, {nl}>"
+    if kind == 2:
+        return f"los_klane<{pol}, {nl}, {ch}>"
+    return f"los_integrate_stream{'_q' if pers else ''}<{pol}, {nk}, {ch}>"
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter table, or None."""
+    try:
+        with open(TRAFFIC_TABLE.format(workload=workload)) as fh:
+            tab = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    ent = tab.get("kernels", {}).get(kernel)
+    return None if ent is None else float(ent["traffic_bytes"])
 
 
 def select_pairs(cfg: L.Config, n: int):
@@ -345,6 +372,7 @@ def main(argv=None):
             res["hbm_gbs"] = res["alg_bytes"] / (los_ms * 1e-3) / 1e9
             res["valu_tflops"] = flop_live / (los_ms * 1e-3) / 1e12
         kind, nk, ch, pers = eng.get_shape(FORMS[form])
+        res["kernel"] = kernel_name(form, (kind, nk, ch, pers), max(cfg.l_max_los_t, cfg.l_max_los_p) + 1)
         res["tile_shape"] = ({"kernel": "sweep", "nk": nk} if kind == 1 else
                              {"kernel": "k-lane", "l_per_lane": ch} if kind == 2 else
                              {"kernel": "workgroup per k3"} if form == "fp64" else
@@ -391,8 +419,9 @@ def main(argv=None):
         "config": {"workload": f"{args.workload}: {DESCR[args.workload]}", "form": args.form,
                    "elements": n_el, "pairs": len(pairs)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": KERNELS[args.form], "kernel_ms": head["kernel_ms"],
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.workload, head["kernel"]),
+                     "kernel": head["kernel"], "kernel_ms": head["kernel_ms"],
                      "alg_bytes": alg_bytes,
                      "valu": {"achieved": head.get("valu_tflops", 0.0),
                               "peak": VALU_PEAK_TFLOPS[args.form], "unit": "TFLOP/s",
