@@ -9,6 +9,7 @@
 #define LOS_MAX_MI 6      /* entries of the m-list */
 #define LOS_NC_MAX 256    /* coarse k3 values per pair handled by the spline kernel */
 #define LOS_EXACT_BLOCK_MAX 512 /* (l,m) lanes per workgroup of the exact LOS kernel */
+#define LOS_BUF_MAX 0xFFFFF000u /* largest table read through a raw buffer resource */
 
 /* One (k1,k2) pair as laid out on the device. */
 typedef struct {
@@ -108,7 +109,8 @@ typedef struct {
   /* Sweep kernels (FMA-FP64 and FP32-compensated forms). One wave handles one
      m and 64 consecutive l: lane slot lp = mi * n_lp + il (n_lp = l count
      rounded up to 64). */
-  const void *Jp;        /* J re-laid out [jt][L][x][lp] (FP64 or FP32) */
+  const void *Jp;        /* J re-laid out [jt][L][x][lp] (FP64 pairs or FP32) */
+  unsigned Jp_bytes;     /* size of the table at Jp (read with 32-bit byte offsets) */
   const int *xminp;      /* [jt][L][lp] first live x index, INT_MAX for empty slots */
   const int *lm_of_lp;   /* (l,m) offset of each slot, -1 for empty slots */
   int n_lp;              /* l slots per m */

@@ -694,16 +694,28 @@ int enqueue_prep(los_engine *e, char *err)
 // one-workgroup-per-k3 kernel (kind 0) and the k-lane kernel (kind 2), both
 // bit-identical to the reference; the fast forms every stream, sweep and
 // k-lane shape that fits.
-static int form_shapes(int form, int tp2_size, los_shape *out, int cap)
+// Whether the sweep kernels can address the batch's J table (32-bit offsets).
+bool sweep_fits(const los_engine *e, int form)
 {
+  const double bytes = (double)e->nJp * (form == LOS_FORM_FP32C ? sizeof(float) : 2 * sizeof(double));
+  return bytes <= (double)LOS_BUF_MAX;
+}
+
+static int form_shapes(const los_engine *e, int form, los_shape *out, int cap)
+{
+  los_shape all[32];
+  int n = 0;
   if (form == LOS_FORM_FP64) {
-    int n = 0;
-    if (n < cap) out[n++] = los_shape{0, 1, 0, 0};
-    for (int nk = 1; nk <= 4 && n < cap; nk *= 2) out[n++] = los_shape{1, nk, 0, 0};
-    for (int b = 1; b <= 4 && n < cap; b *= 2) out[n++] = los_shape{2, 64, b, 0};
-    return n;
+    all[n++] = los_shape{0, 1, 0, 0};
+    for (int nk = 1; nk <= 4; nk *= 2) all[n++] = los_shape{1, nk, 0, 0};
+    for (int b = 1; b <= 4; b *= 2) all[n++] = los_shape{2, 64, b, 0};
+  } else {
+    n = los_shapes(form == LOS_FORM_FP32C, e->loaded ? e->P.tp2_size : 1, all, 32);
   }
-  return los_shapes(form == LOS_FORM_FP32C, tp2_size, out, cap);
+  int k = 0;
+  for (int i = 0; i < n && k < cap; ++i)
+    if (all[i].kind != 1 || sweep_fits(e, form)) out[k++] = all[i];
+  return k;
 }
 
 // The LOS kernel of the current form with shape sh, on work list P (tiles and
@@ -728,6 +740,7 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   }
   if (e->form == LOS_FORM_FP64 && sh->kind == 1) {
     P->Jp = e->dJp2[0];
+    P->Jp_bytes = (unsigned)(e->nJp * 2 * sizeof(double));
     split_xcd(tiles, cost, P->xcd_tile);
     HIPCHK(los_launch_sweep_exact(P, sh->nk, s), err);
     return _SUCCESS_;
@@ -740,6 +753,7 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   split_xcd(tiles, cost, P->xcd_tile);
   if (sh->kind == 1) {
     P->Jp = fp32 ? (const void *)e->dJp32 : e->dJp2[0];
+    P->Jp_bytes = (unsigned)(e->nJp * (fp32 ? sizeof(float) : 2 * sizeof(double)));
     HIPCHK(los_launch_sweep(fp32, P, sh->nk, s), err);
   } else {
     HIPCHK(los_launch_stream(fp32, P, sh, e->n_cu, s), err);
@@ -803,6 +817,13 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
 {
   if (sh == nullptr || sh->kind != 1) return _SUCCESS_;
   const int fp32 = e->form == LOS_FORM_FP32C;
+  if (!sweep_fits(e, e->form)) {
+    snprintf(err, _ERRORMSGSIZE_,
+             "sweep shape: the J table of this batch (%zu entries) exceeds the 32-bit offsets "
+             "of the sweep kernels; use another kernel organisation",
+             e->nJp);
+    return _FAILURE_;
+  }
   if (fp32 && e->dJp32 == nullptr) {
     HIPCHK(hipMalloc((void **)&e->dJp32, std::max<size_t>(1, e->nJp) * sizeof(float)), err);
     HIPCHK(los_launch_round_j32(e->dJp, e->dJp32, (long)e->nJp, e->stream), err);
@@ -914,7 +935,7 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
   if (reps < 1) reps = 1;
   if (ensure_j32(e, err) || enqueue_prep(e, err)) return _FAILURE_;
   los_shape cand[32];
-  const int nc = form_shapes(form, e->P.tp2_size, cand, 32);
+  const int nc = form_shapes(e, form, cand, 32);
   if (nc == 0) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: no tile shape fits %d source types in LDS",
              e->P.tp2_size);
@@ -953,13 +974,14 @@ extern "C" int los_engine_set_shape(los_engine *e, int form, int kind, int nk, i
     return _FAILURE_;
   }
   los_shape cand[32];
-  const int nc = form_shapes(form, e->loaded ? e->P.tp2_size : 1, cand, 32);
+  const int nc = form_shapes(e, form, cand, 32);
   for (int i = 0; i < nc; ++i)
     if (cand[i].kind == kind &&
-        (kind == 2 ? cand[i].ch == ch
-                   : (form == LOS_FORM_FP64 ||
-                      (cand[i].nk == nk &&
-                       (kind == 1 || (cand[i].ch == ch && cand[i].persistent == (persistent != 0))))))) {
+        (kind == 2   ? cand[i].ch == ch
+         : kind == 1 ? cand[i].nk == nk
+                     : (form == LOS_FORM_FP64 ||
+                        (cand[i].nk == nk && cand[i].ch == ch &&
+                         cand[i].persistent == (persistent != 0))))) {
       e->shape[form] = cand[i];
       e->shape_set[form] = true;
       drop_graph(e);

@@ -159,6 +159,7 @@ std::vector<los_tile> make_tiles(const std::vector<int> &job_end, int nk);
 int enqueue_prep(los_engine *e, char *err);
 int ensure_j32(los_engine *e, char *err);
 int current_shape(los_engine *e, const los_shape **sh, char *err);
+bool sweep_fits(const los_engine *e, int form);
 int ensure_sweep(los_engine *e, const los_shape *sh, char *err);
 int make_sweep_tables(los_engine *e, char *err);
 int make_klane_tables(los_engine *e, char *err);

@@ -46,26 +46,42 @@ struct alignas(2 * sizeof(R)) SweepPair {
   R j0, j1;
 };
 
-// J[x] and J[x+1] of a row at lane slot lp. FP64: from the pair table (one
-// load); FP32: from the plain sweep table (two loads, which share cache lines
-// with the neighbouring time steps).
-template <class R>
-__device__ __forceinline__ void sweep_row(const void *Jb, long rowoff, long o0, int NLP, R &j0, R &j1);
-template <>
-__device__ __forceinline__ void sweep_row<double>(const void *Jb, long rowoff, long o0, int NLP,
-                                                 double &j0, double &j1)
+// The J table is read through a raw buffer resource: a load adds the lane's
+// byte offset (lane slot) to a wave-uniform 32-bit byte offset (row and x), so
+// the address costs one scalar add and the row offsets fit in single SGPRs.
+// The table must be smaller than LOS_BUF_MAX bytes (the host checks).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sweep_rsrc(const void *base, unsigned bytes)
 {
-  const SweepPair<double> w = ((const SweepPair<double> *)Jb)[rowoff + o0];
-  j0 = w.j0;
-  j1 = w.j1;
+  // word 3 of a gfx9-family raw buffer descriptor: 32-bit data, no swizzle
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+
+// J[x] and J[x+1] of a row at one lane slot, `off` = byte offset of (row, x).
+// FP64: one 16-byte load from the pair table; FP32: two loads from the plain
+// table (`xstep` bytes apart), which share cache lines with the neighbouring
+// time steps. The pair is returned whole, so a conditional load needs no
+// register moves (and no wait) where the branches join.
+template <class R>
+__device__ __forceinline__ SweepPair<R> sweep_row(__amdgpu_buffer_rsrc_t J, unsigned lane,
+                                                  unsigned off, unsigned xstep);
+template <>
+__device__ __forceinline__ SweepPair<double> sweep_row<double>(__amdgpu_buffer_rsrc_t J,
+                                                               unsigned lane, unsigned off,
+                                                               unsigned)
+{
+  return __builtin_bit_cast(SweepPair<double>,
+                            __builtin_amdgcn_raw_buffer_load_b128(J, lane, off, 0));
 }
 template <>
-__device__ __forceinline__ void sweep_row<float>(const void *Jb, long rowoff, long o0, int NLP,
-                                                float &j0, float &j1)
+__device__ __forceinline__ SweepPair<float> sweep_row<float>(__amdgpu_buffer_rsrc_t J,
+                                                             unsigned lane, unsigned off,
+                                                             unsigned xstep)
 {
-  const float *r = (const float *)Jb + rowoff + o0;
-  j0 = r[0];
-  j1 = r[NLP];
+  SweepPair<float> p;
+  p.j0 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(J, lane, off, 0));
+  p.j1 = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(J, lane, off + xstep, 0));
+  return p;
 }
 
 // Pair table [row][x][lp] from the sweep table [row][x][lp] (x + 1 = X reads 0).
@@ -188,18 +204,27 @@ los_sweep(los_params P)
       if (P.jEB >= 0) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
     }
   }
-  // FP64: pair table, FP32: plain table (both [row][x][lp]), offset to this lane
-  const void *Jb = sizeof(R) == 8 ? (const void *)((const SweepPair<double> *)P.Jp + lp)
-                                  : (const void *)((const float *)P.Jp + lp);
-  const long rowstep = (long)P.X * NLP;
+  // FP64: pair table, FP32: plain table (both [row][x][lp])
+  const __amdgpu_buffer_rsrc_t Jr = sweep_rsrc(P.Jp, P.Jp_bytes);
+  constexpr unsigned esz = sizeof(R) == 8 ? 16u : 4u;  // bytes of one x slot of one lane
+  const unsigned lane = (unsigned)lp * esz;
+  const unsigned xstep = (unsigned)NLP * esz;
+  const unsigned rowb = (unsigned)P.X * xstep;
   const SweepIXA<R> *__restrict__ ixa = (const SweepIXA<R> *)P.ixa;
   const SweepS<R> *__restrict__ sf = (const SweepS<R> *)P.sf;
   const bool doT = P.has_T, doP = P.has_E || P.has_B, mix = P.jEB >= 0;
+  const unsigned bT = doT ? (unsigned)(P.jTT * P.L_size) * rowb : 0u;
+  const unsigned bE = doP ? (unsigned)(P.jEE * P.L_size) * rowb : 0u;
+  const unsigned bQ = mix ? (unsigned)(P.jEB * P.L_size) * rowb : 0u;
   const R emix = m != 0 ? (R)1 : (R)0;  // E takes the B->E mixing only for m != 0
 
   A aT[NK], aE[NK], aB[NK];
 #pragma unroll
   for (int j = 0; j < NK; ++j) aT[j] = aE[j] = aB[j] = A{};
+  // J rows at the current index_x; rows this wave never reads stay zero
+  SweepPair<R> tv[NL], ev[NL], qv[NL];
+#pragma unroll
+  for (int L = 0; L < NL; ++L) tv[L] = ev[L] = qv[L] = SweepPair<R>{(R)0, (R)0};
 
   for (int t0 = 0; t0 < tend; t0 += FOLD) {
     const int t1 = min(tend, t0 + FOLD);
@@ -218,19 +243,17 @@ los_sweep(los_params P)
       // the NK elements in turn; the J rows are loaded again only when an
       // element's index_x differs from the previous one's (neighbouring k3
       // usually share it), so one set of row registers serves the tile
-      R t0v[NL], t1v[NL], e0v[NL], e1v[NL], q0v[NL], q1v[NL];
 #pragma unroll
       for (int j = 0; j < NK; ++j) {
         if (j == 0 || ix[j] != ix[j - 1]) {
-          const long o0 = (long)ix[j] * NLP;
+          const unsigned ox = (unsigned)ix[j] * xstep;
 #pragma unroll
           for (int L = 0; L < NL; ++L) {
-            t0v[L] = t1v[L] = e0v[L] = e1v[L] = q0v[L] = q1v[L] = 0;
             if (m > L) continue;
-            if (doT && L <= P.LmaxT) sweep_row(Jb, (long)(P.jTT * P.L_size + L) * rowstep, o0, NLP, t0v[L], t1v[L]);
+            if (doT && L <= P.LmaxT) tv[L] = sweep_row<R>(Jr, lane, bT + L * rowb + ox, xstep);
             if (doP && L <= P.LmaxP) {
-              sweep_row(Jb, (long)(P.jEE * P.L_size + L) * rowstep, o0, NLP, e0v[L], e1v[L]);
-              if (mix) sweep_row(Jb, (long)(P.jEB * P.L_size + L) * rowstep, o0, NLP, q0v[L], q1v[L]);
+              ev[L] = sweep_row<R>(Jr, lane, bE + L * rowb + ox, xstep);
+              if (mix) qv[L] = sweep_row<R>(Jr, lane, bQ + L * rowb + ox, xstep);
             }
           }
         }
@@ -240,13 +263,13 @@ los_sweep(los_params P)
           if (m > L) continue;
           const SweepS<R> s = sp[L];
           if (doT && L <= P.LmaxT) {
-            R Jv = t1v[L] + a[j] * (t0v[L] - t1v[L]);
+            R Jv = tv[L].j1 + a[j] * (tv[L].j0 - tv[L].j1);
             Jv = ix[j] >= xmT[L] ? Jv : (R)0;
             pT[j] += Jv * s.T;
           }
           if (doP && L <= P.LmaxP) {
-            R Je = e1v[L] + a[j] * (e0v[L] - e1v[L]);
-            R Jm = q1v[L] + a[j] * (q0v[L] - q1v[L]);
+            R Je = ev[L].j1 + a[j] * (ev[L].j0 - ev[L].j1);
+            R Jm = qv[L].j1 + a[j] * (qv[L].j0 - qv[L].j1);
             Je = ix[j] >= xmE[L] ? Je : (R)0;
             Jm = ix[j] >= xmM[L] ? Jm : (R)0;
             pE[j] += Je * s.E + Jm * (emix * s.B);
@@ -357,16 +380,25 @@ los_sweep_exact(los_params P)
       if (P.jEB >= 0 && P.has_B) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
     }
   }
-  const void *Jb = (const void *)((const SweepPair<double> *)P.Jp + lp);
-  const long rowstep = (long)P.X * NLP;
+  const __amdgpu_buffer_rsrc_t Jr = sweep_rsrc(P.Jp, P.Jp_bytes);
+  const unsigned lane = (unsigned)lp * 16u;
+  const unsigned xstep = (unsigned)NLP * 16u;
+  const unsigned rowb = (unsigned)P.X * xstep;
   const SweepIXA<double> *__restrict__ ixa = (const SweepIXA<double> *)P.ixa;
   const SweepS<double> *__restrict__ sf = (const SweepS<double> *)P.sf;
   const bool doT = P.has_T && P.jTT >= 0, doP = (P.has_E || P.has_B) && P.jEE >= 0;
   const int jEB = P.jEB >= 0 ? P.jEB : P.jEE;
+  const unsigned bT = doT ? (unsigned)(P.jTT * P.L_size) * rowb : 0u;
+  const unsigned bE = doP ? (unsigned)(P.jEE * P.L_size) * rowb : 0u;
+  const unsigned bQ = doP ? (unsigned)(jEB * P.L_size) * rowb : 0u;
 
   double accT[NK], accEd[NK], accEm[NK], accBd[NK], accBm[NK];
 #pragma unroll
   for (int j = 0; j < NK; ++j) accT[j] = accEd[j] = accEm[j] = accBd[j] = accBm[j] = 0;
+  // J rows at the current index_x (rows this wave never reads stay zero)
+  SweepPair<double> tv[NL], ev[NL], qv[NL];
+#pragma unroll
+  for (int L = 0; L < NL; ++L) tv[L] = ev[L] = qv[L] = SweepPair<double>{0., 0.};
   for (int t = 0; t < tend; ++t) {
     const double dt = P.dtau[t];
     int ix[NK];
@@ -377,19 +409,17 @@ los_sweep_exact(los_params P)
       ix[j] = v.ix;
       a[j] = v.a;
     }
-    double t0v[NL], t1v[NL], e0v[NL], e1v[NL], q0v[NL], q1v[NL];
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
       if (j == 0 || ix[j] != ix[j - 1]) {
-        const long o0 = (long)ix[j] * NLP;
+        const unsigned ox = (unsigned)ix[j] * xstep;
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
-          t0v[L] = t1v[L] = e0v[L] = e1v[L] = q0v[L] = q1v[L] = 0;
           if (m > L) continue;
-          if (doT && L <= P.LmaxT) sweep_row(Jb, (long)(P.jTT * P.L_size + L) * rowstep, o0, NLP, t0v[L], t1v[L]);
+          if (doT && L <= P.LmaxT) tv[L] = sweep_row<double>(Jr, lane, bT + L * rowb + ox, xstep);
           if (doP && L <= P.LmaxP) {
-            sweep_row(Jb, (long)(P.jEE * P.L_size + L) * rowstep, o0, NLP, e0v[L], e1v[L]);
-            sweep_row(Jb, (long)(jEB * P.L_size + L) * rowstep, o0, NLP, q0v[L], q1v[L]);
+            ev[L] = sweep_row<double>(Jr, lane, bE + L * rowb + ox, xstep);
+            qv[L] = sweep_row<double>(Jr, lane, bQ + L * rowb + ox, xstep);
           }
         }
       }
@@ -399,7 +429,7 @@ los_sweep_exact(los_params P)
         double integrand = 0;
 #pragma unroll
         for (int L = 0; L < NL; ++L)
-          if (ix[j] >= xmT[L]) integrand += (aJ * t0v[L] + (1 - aJ) * t1v[L]) * sp[L].T;
+          if (ix[j] >= xmT[L]) integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sp[L].T;
         accT[j] += integrand * dt;
       }
       if (doP) {
@@ -408,12 +438,12 @@ los_sweep_exact(los_params P)
         for (int L = 0; L < NL; ++L) {
           const SweepS<double> s = sp[L];
           if (ix[j] >= xmE[L]) {
-            const double Jv = aJ * e0v[L] + (1 - aJ) * e1v[L];
+            const double Jv = aJ * ev[L].j0 + (1 - aJ) * ev[L].j1;
             if (P.has_E) iEd += Jv * s.E;
             if (P.has_B) iBd += Jv * s.B;
           }
           if (ix[j] >= xmM[L]) {
-            const double Jv = aJ * q0v[L] + (1 - aJ) * q1v[L];
+            const double Jv = aJ * qv[L].j0 + (1 - aJ) * qv[L].j1;
             if (m != 0) iEm += Jv * s.B;
             iBm += Jv * s.E;
           }
