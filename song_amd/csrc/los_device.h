@@ -38,7 +38,9 @@ typedef struct {
 /* Kernel organisation and tile shape of the fast forms:
    kind 0: "stream" kernel, nk fine k3 per workgroup, ch time steps per LDS
            chunk, persistent = 1 for the chip-sized grid fed by a work queue;
-   kind 1: "sweep" kernel, one wave per (tile of nk fine k3, m, 64 l);
+   kind 1: "sweep" kernel, one wave per (tile of nk fine k3, m, 64 l) item,
+           persistent = 1 for the chip-sized grid of waves striding over the
+           items of their XCD;
    kind 2: "k-lane" kernel, one wave per ((l,m), 64 neighbouring fine k3);
    kind 0 of the plain FP64 form is its one-workgroup-per-k3 kernel. */
 typedef struct {

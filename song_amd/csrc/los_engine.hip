@@ -681,19 +681,12 @@ int enqueue_prep(los_engine *e, char *err)
 {
   const int tp2 = e->P.tp2_size;
   const int np = (int)e->pairs.size();
-  HIPCHK(los_launch_transpose(e->dpairs, np, e->nc_max, e->draw, e->dsrc, e->ntau, tp2, e->stream),
+  HIPCHK(los_launch_relayout_spline(e->dpairs, np, e->nc_max, e->dkc, e->dspl, e->draw, e->dsrc,
+                                    e->ddd, e->ntau, tp2, e->P.cubic_k3, e->stream),
          err);
-  if (e->P.cubic_k3)
-    HIPCHK(los_launch_spline_tau(e->dpairs, np, e->dkc, e->dspl, e->dsrc, e->ddd, e->ntau, tp2,
-                                 e->stream),
-           err);
   return _SUCCESS_;
 }
 
-// Candidate kernel shapes of a form: the plain FP64 form has its
-// one-workgroup-per-k3 kernel (kind 0) and the k-lane kernel (kind 2), both
-// bit-identical to the reference; the fast forms every stream, sweep and
-// k-lane shape that fits.
 // Whether the sweep kernels can address the batch's J table (32-bit offsets).
 bool sweep_fits(const los_engine *e, int form)
 {
@@ -701,22 +694,33 @@ bool sweep_fits(const los_engine *e, int form)
   return bytes <= (double)LOS_BUF_MAX;
 }
 
+// Candidate kernel shapes of a form: the plain FP64 form has its
+// one-workgroup-per-k3 kernel (kind 0), the sweep (kind 1) and the k-lane
+// kernel (kind 2), all bit-identical to the reference; the fast forms every
+// stream, sweep and k-lane shape that fits.
 static int form_shapes(const los_engine *e, int form, los_shape *out, int cap)
 {
-  los_shape all[32];
+  los_shape all[48];
   int n = 0;
   if (form == LOS_FORM_FP64) {
     all[n++] = los_shape{0, 1, 0, 0};
-    for (int nk = 1; nk <= 4; nk *= 2) all[n++] = los_shape{1, nk, 0, 0};
+    for (int nk = 1; nk <= 4; nk *= 2)
+      for (int p = 0; p < 2; ++p) all[n++] = los_shape{1, nk, 0, p};
     for (int b = 1; b <= 4; b *= 2) all[n++] = los_shape{2, 64, b, 0};
   } else {
-    n = los_shapes(form == LOS_FORM_FP32C, e->loaded ? e->P.tp2_size : 1, all, 32);
+    n = los_shapes(form == LOS_FORM_FP32C, e->loaded ? e->P.tp2_size : 1, all, 48);
   }
   int k = 0;
   for (int i = 0; i < n && k < cap; ++i)
     if (all[i].kind != 1 || sweep_fits(e, form)) out[k++] = all[i];
   return k;
 }
+
+// Shapes the autotuner times: all but the persistent sweep, which measured
+// slower than the dispatched sweep on every workload tried (the hardware's
+// dynamic assignment of finished waves balances better than a fixed stride);
+// it stays available through los_engine_set_shape.
+static bool tuned_shape(const los_shape &s) { return !(s.kind == 1 && s.persistent); }
 
 // The LOS kernel of the current form with shape sh, on work list P (tiles and
 // cost: host copies of its tiles and per-position costs). A sweep or k-lane
@@ -742,7 +746,7 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
     P->Jp = e->dJp2[0];
     P->Jp_bytes = (unsigned)(e->nJp * 2 * sizeof(double));
     split_xcd(tiles, cost, P->xcd_tile);
-    HIPCHK(los_launch_sweep_exact(P, sh->nk, s), err);
+    HIPCHK(los_launch_sweep_exact(P, sh->nk, sh->persistent, e->n_cu, s), err);
     return _SUCCESS_;
   }
   if (e->form == LOS_FORM_FP64) {
@@ -754,7 +758,7 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   if (sh->kind == 1) {
     P->Jp = fp32 ? (const void *)e->dJp32 : e->dJp2[0];
     P->Jp_bytes = (unsigned)(e->nJp * (fp32 ? sizeof(float) : 2 * sizeof(double)));
-    HIPCHK(los_launch_sweep(fp32, P, sh->nk, s), err);
+    HIPCHK(los_launch_sweep(fp32, P, sh->nk, sh->persistent, e->n_cu, s), err);
   } else {
     HIPCHK(los_launch_stream(fp32, P, sh, e->n_cu, s), err);
   }
@@ -934,8 +938,11 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
   }
   if (reps < 1) reps = 1;
   if (ensure_j32(e, err) || enqueue_prep(e, err)) return _FAILURE_;
-  los_shape cand[32];
-  const int nc = form_shapes(e, form, cand, 32);
+  los_shape all[48], cand[48];
+  const int na = form_shapes(e, form, all, 48);
+  int nc = 0;
+  for (int i = 0; i < na; ++i)
+    if (tuned_shape(all[i])) cand[nc++] = all[i];
   if (nc == 0) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: no tile shape fits %d source types in LDS",
              e->P.tp2_size);
@@ -973,12 +980,12 @@ extern "C" int los_engine_set_shape(los_engine *e, int form, int kind, int nk, i
     snprintf(err, _ERRORMSGSIZE_, "los_engine_set_shape: unknown form %d", form);
     return _FAILURE_;
   }
-  los_shape cand[32];
-  const int nc = form_shapes(e, form, cand, 32);
+  los_shape cand[48];
+  const int nc = form_shapes(e, form, cand, 48);
   for (int i = 0; i < nc; ++i)
     if (cand[i].kind == kind &&
         (kind == 2   ? cand[i].ch == ch
-         : kind == 1 ? cand[i].nk == nk
+         : kind == 1 ? (cand[i].nk == nk && cand[i].persistent == (persistent != 0))
                      : (form == LOS_FORM_FP64 ||
                         (cand[i].nk == nk && cand[i].ch == ch &&
                          cand[i].persistent == (persistent != 0))))) {

@@ -12,21 +12,21 @@
 #include "los_device.h"
 #include "los_internal.h"
 
-extern "C" hipError_t los_launch_transpose(const los_pair *pairs, int npairs, int nc_max,
-                                           const double *raw, double *out, int ntau, int tp2,
-                                           hipStream_t s);
-extern "C" hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double *kc,
-                                            const double *spl, const double *src, double *dd,
-                                            int ntau, int tp2, hipStream_t s);
+extern "C" hipError_t los_launch_relayout_spline(const los_pair *pairs, int npairs, int nc_max,
+                                                 const double *kc, const double *spl,
+                                                 const double *raw, double *src, double *dd,
+                                                 int ntau, int tp2, int cubic, hipStream_t s);
 extern "C" hipError_t los_launch_integrate_exact(const los_params *P, int nthreads, hipStream_t s);
 extern "C" hipError_t los_launch_stream(int fp32, const los_params *P, const los_shape *sh,
                                         int n_cu, hipStream_t s);
 extern "C" hipError_t los_stream_set_attrs(int tp2_size);
 extern "C" int los_shapes(int fp32, int tp2_size, los_shape *out, int cap);
 extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s);
-extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s);
+extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int n_cu,
+                                       hipStream_t s);
 extern "C" hipError_t los_launch_sweep_prep_exact(const los_params *P, hipStream_t s);
-extern "C" hipError_t los_launch_sweep_exact(const los_params *P, int nk, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_cu,
+                                             hipStream_t s);
 extern "C" hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, long rows, int X,
                                              int NLP, hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s);

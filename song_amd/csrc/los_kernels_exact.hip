@@ -23,86 +23,99 @@
 extern "C" {
 
 // ---------------------------------------------------------------------------
-// Source re-layout: each (pair, tp) block [tau][kc] (k3 fastest, as SONG
-// stores it) becomes [kc][tau], so that the lanes of a wave, which run over
-// tau, read contiguous memory. 32x32 tiles through LDS.
+// Source re-layout and k3 spline in one pass. A workgroup (one wave) takes
+// LOS_RS_TW consecutive tau of one (pair, tp) block: it stages their
+// [tau][kc] rows (k3 fastest, as SONG stores them, so the read is one
+// contiguous run) in LDS, writes them out as [kc][tau] (tau-contiguous for
+// the table kernels), then each lane solves the spline of its tau column in
+// place in LDS and the second derivatives are written out as [kc][tau].
+// The y-independent recurrence values sig_i, p_i and gamma_i = (sig_i-1)/p_i
+// of the tridiagonal sweep are computed on the host with the same operations
+// and passed per pair (spl[3*kc_off + {0,1,2}*nc + i]); the column's u_i
+// overwrite its y_i once y_i is consumed, and the back substitution
+// overwrites u_i. LDS row stride `ls` is odd (in doubles), so lanes that walk
+// their own columns hit distinct banks.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-los_transpose_sources(const los_pair *pairs, const double *raw, double *out, int ntau, int tp2)
+#define LOS_RS_TW 64
+
+__global__ void __launch_bounds__(LOS_RS_TW)
+los_relayout_spline(const los_pair *pairs, const double *kc_all, const double *spl,
+                    const double *raw, double *src, double *dd, int ntau, int tp2, int nblk,
+                    int cubic, int ls)
 {
-  __shared__ double tile[32][33];
-  const int mat = blockIdx.z;  // pair * tp2 + tp
+  HIP_DYNAMIC_SHARED(double, tile)  // [LOS_RS_TW][ls]
+  const int mat = blockIdx.x / nblk;  // pair * tp2 + tp
   const int p = mat / tp2, tp = mat - p * tp2;
-  const int nc = pairs[p].nc;
-  const int kc0 = blockIdx.x * 32, t0 = blockIdx.y * 32;
-  if (kc0 >= nc) return;
-  const long base = pairs[p].src_off + (long)tp * ntau * nc;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int r = ty; r < 32; r += 8) {
-    const int t = t0 + r, kc = kc0 + tx;
-    if (t < ntau && kc < nc) tile[r][tx] = raw[base + (long)t * nc + kc];
+  const int t0 = (blockIdx.x - mat * nblk) * LOS_RS_TW;
+  const int n = pairs[p].nc;
+  const int nt = min(LOS_RS_TW, ntau - t0);
+  const long base = pairs[p].src_off + (long)tp * ntau * n;
+  const int lane = threadIdx.x;
+
+  // rows t0 .. t0+nt-1 of [tau][kc] are contiguous: nt * n doubles, read in
+  // batches of LOS_RS_B loads per lane so that they are all in flight at once
+  {
+    constexpr int LOS_RS_B = 16;
+    const double *r = raw + base + (long)t0 * n;
+    const int total = nt * n;
+    for (int i0 = 0; i0 < total; i0 += LOS_RS_TW * LOS_RS_B) {
+      double v[LOS_RS_B];
+#pragma unroll
+      for (int u = 0; u < LOS_RS_B; ++u) {
+        const int i = i0 + u * LOS_RS_TW + lane;
+        v[u] = i < total ? r[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < LOS_RS_B; ++u) {
+        const int i = i0 + u * LOS_RS_TW + lane;
+        if (i < total) {
+          const int t = i / n;
+          tile[t * ls + (i - t * n)] = v[u];
+        }
+      }
+    }
   }
   __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    const int kc = kc0 + r, t = t0 + tx;
-    if (t < ntau && kc < nc) out[base + (long)kc * ntau + t] = tile[tx][r];
-  }
-}
+  const bool live = lane < nt;
+  double *so = src + base + t0 + lane;
+  if (live)
+    for (int kc = 0; kc < n; ++kc) so[(long)kc * ntau] = tile[lane * ls + kc];
+  if (!cubic) return;
 
-// ---------------------------------------------------------------------------
-// Spline second derivatives in k3 on the tau-contiguous layout: one thread
-// per (pair, tp, tau) column, consecutive threads on consecutive tau. The
-// y-independent recurrence values sig_i, p_i and gamma_i = (sig_i-1)/p_i of the
-// tridiagonal sweep are computed on the host with the same operations and
-// passed per pair (spl[3*kc_off + {0,1,2}*nc + i]); u_i is kept in dd until
-// the back substitution overwrites it.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-los_spline_k3_tau(const los_pair *pairs, long ncols, const double *kc_all,
-                  const double *spl, const double *src, double *dd, int ntau, int tp2_size)
-{
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
-  const long per_pair = (long)tp2_size * ntau;
-  const int p = (int)(c / per_pair);
-  const long r = c - (long)p * per_pair;
-  const int tp = (int)(r / ntau);
-  const int tau = (int)(r - (long)tp * ntau);
-  const int n = pairs[p].nc;
-  const double *x = kc_all + pairs[p].kc_off;
-  const double *sig = spl + 3L * pairs[p].kc_off;
-  const double *pv = sig + n;
-  const double *gam = pv + n;
-  const long base = pairs[p].src_off + (long)tp * n * ntau + tau;
-  const double *y = src + base;
-  double *d = dd + base;
-  const long s = ntau;
-
-  double y0 = y[0], y1 = y[s], y2 = y[2 * s];
-  double d0 = ((x[2] - x[0]) * (x[2] - x[0]) * (y1 - y0) - (x[1] - x[0]) * (x[1] - x[0]) * (y2 - y0)) /
-              ((x[2] - x[0]) * (x[1] - x[0]) * (x[2] - x[1]));
-  double u = (3.0 / (x[1] - x[0])) * ((y1 - y0) / (x[1] - x[0]) - d0);
-  d[0] = u;
-  double ym = y0, yc = y1;
-  for (int i = 1; i < n - 1; ++i) {
-    const double yn = y[(long)(i + 1) * s];
-    double t = (yn - yc) / (x[i + 1] - x[i]) - (yc - ym) / (x[i] - x[i - 1]);
-    u = (6.0 * t / (x[i + 1] - x[i - 1]) - sig[i] * u) / pv[i];
-    d[(long)i * s] = u;
-    ym = yc;
-    yc = yn;
-  }
-  const int a = n - 3, b = n - 2, e = n - 1;
-  const double ya = y[(long)a * s], yb = y[(long)b * s], ye = y[(long)e * s];
-  double dn = ((x[e] - x[a]) * (x[e] - x[a]) * (ye - yb) - (x[e] - x[b]) * (x[e] - x[b]) * (ye - ya)) /
-              ((x[e] - x[a]) * (x[e] - x[b]) * (x[b] - x[a]));
-  double qn = 0.5;
-  double un = (3.0 / (x[e] - x[b])) * (dn - (ye - yb) / (x[e] - x[b]));
-  double next = (un - qn * u) / (qn * gam[n - 2] + 1.0);
-  d[(long)e * s] = next;
-  for (int k = n - 2; k >= 0; --k) {
-    next = gam[k] * next + d[(long)k * s];
-    d[(long)k * s] = next;
+  if (live) {
+    const double *x = kc_all + pairs[p].kc_off;
+    const double *sig = spl + 3L * pairs[p].kc_off;
+    const double *pv = sig + n;
+    const double *gam = pv + n;
+    double *y = tile + lane * ls;  // y_i, then u_i, then the second derivatives
+    const int a = n - 3, b = n - 2, e = n - 1;
+    const double ya = y[a], yb = y[b], ye = y[e];
+    const double y0 = y[0], y1 = y[1], y2 = y[2];
+    double d0 = ((x[2] - x[0]) * (x[2] - x[0]) * (y1 - y0) - (x[1] - x[0]) * (x[1] - x[0]) * (y2 - y0)) /
+                ((x[2] - x[0]) * (x[1] - x[0]) * (x[2] - x[1]));
+    double u = (3.0 / (x[1] - x[0])) * ((y1 - y0) / (x[1] - x[0]) - d0);
+    y[0] = u;
+    double ym = y0, yc = y1;
+    for (int i = 1; i < n - 1; ++i) {
+      const double yn = y[i + 1];
+      double t = (yn - yc) / (x[i + 1] - x[i]) - (yc - ym) / (x[i] - x[i - 1]);
+      u = (6.0 * t / (x[i + 1] - x[i - 1]) - sig[i] * u) / pv[i];
+      y[i] = u;
+      ym = yc;
+      yc = yn;
+    }
+    double dn = ((x[e] - x[a]) * (x[e] - x[a]) * (ye - yb) - (x[e] - x[b]) * (x[e] - x[b]) * (ye - ya)) /
+                ((x[e] - x[a]) * (x[e] - x[b]) * (x[b] - x[a]));
+    double qn = 0.5;
+    double un = (3.0 / (x[e] - x[b])) * (dn - (ye - yb) / (x[e] - x[b]));
+    double next = (un - qn * u) / (qn * gam[n - 2] + 1.0);
+    y[e] = next;
+    for (int k = n - 2; k >= 0; --k) {
+      next = gam[k] * next + y[k];
+      y[k] = next;
+    }
+    double *dout = dd + base + t0 + lane;
+    for (int kc = 0; kc < n; ++kc) dout[(long)kc * ntau] = y[kc];
   }
 }
 
@@ -362,25 +375,25 @@ static hipError_t launch_exact_nl(const los_params *P, dim3 grid, int threads, s
 
 extern "C" {
 
-hipError_t los_launch_transpose(const los_pair *pairs, int npairs, int nc_max, const double *raw,
-                                double *out, int ntau, int tp2, hipStream_t s)
+// Re-layout [tau][kc] -> [kc][tau] of every (pair, tp) block and, if cubic,
+// the k3 spline second derivatives dd in the same layout.
+hipError_t los_launch_relayout_spline(const los_pair *pairs, int npairs, int nc_max,
+                                      const double *kc, const double *spl, const double *raw,
+                                      double *src, double *dd, int ntau, int tp2, int cubic,
+                                      hipStream_t s)
 {
-  if (npairs == 0) return hipSuccess;
-  dim3 grid((nc_max + 31) / 32, (ntau + 31) / 32, (unsigned)(npairs * tp2));
-  hipLaunchKernelGGL(los_transpose_sources, grid, dim3(256), 0, s, pairs, raw, out, ntau, tp2);
-  return hipGetLastError();
-}
-
-hipError_t los_launch_spline_tau(const los_pair *pairs, int npairs, const double *kc,
-                                 const double *spl, const double *src, double *dd, int ntau,
-                                 int tp2, hipStream_t s)
-{
-  const long ncols = (long)npairs * tp2 * ntau;
-  if (ncols == 0) return hipSuccess;
-  const int block = 256;
-  const long grid = (ncols + block - 1) / block;
-  hipLaunchKernelGGL(los_spline_k3_tau, dim3((unsigned)grid), dim3(block), 0, s, pairs, ncols, kc,
-                     spl, src, dd, ntau, tp2);
+  if (npairs == 0 || ntau == 0) return hipSuccess;
+  const int ls = nc_max | 1;
+  const size_t lds = (size_t)LOS_RS_TW * ls * sizeof(double);
+  if (lds > 65536) {
+    const hipError_t r = hipFuncSetAttribute((const void *)los_relayout_spline,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (r != hipSuccess) return r;
+  }
+  const int nblk = (ntau + LOS_RS_TW - 1) / LOS_RS_TW;
+  const long grid = (long)npairs * tp2 * nblk;
+  hipLaunchKernelGGL(los_relayout_spline, dim3((unsigned)grid), dim3(LOS_RS_TW), lds, s, pairs, kc,
+                     spl, raw, src, dd, ntau, tp2, nblk, cubic, ls);
   return hipGetLastError();
 }
 
@@ -390,12 +403,12 @@ hipError_t los_launch_sweep_prep_exact(const los_params *P, hipStream_t s)
   return launch_sweep_prep<double, false>(P, s);
 }
 
-hipError_t los_launch_sweep_exact(const los_params *P, int nk, hipStream_t s)
+hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_cu, hipStream_t s)
 {
   switch (nk) {
-    case 1: return launch_sweep_exact<1>(P, s);
-    case 2: return launch_sweep_exact<2>(P, s);
-    case 4: return launch_sweep_exact<4>(P, s);
+    case 1: return launch_sweep_exact<1>(P, pers != 0, n_cu, s);
+    case 2: return launch_sweep_exact<2>(P, pers != 0, n_cu, s);
+    case 4: return launch_sweep_exact<4>(P, pers != 0, n_cu, s);
     default: return hipErrorInvalidValue;
   }
 }

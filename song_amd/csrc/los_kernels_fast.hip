@@ -96,12 +96,20 @@ hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, long row
   return hipGetLastError();
 }
 
-hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, hipStream_t s)
+hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int n_cu,
+                            hipStream_t s)
 {
+  const bool p = pers != 0;
   switch (nk) {
-    case 1: return fp32 ? launch_sweep<StreamFP32C, 1>(P, s) : launch_sweep<StreamFP64, 1>(P, s);
-    case 2: return fp32 ? launch_sweep<StreamFP32C, 2>(P, s) : launch_sweep<StreamFP64, 2>(P, s);
-    case 4: return fp32 ? launch_sweep<StreamFP32C, 4>(P, s) : launch_sweep<StreamFP64, 4>(P, s);
+    case 1:
+      return fp32 ? launch_sweep<StreamFP32C, 1>(P, p, n_cu, s)
+                  : launch_sweep<StreamFP64, 1>(P, p, n_cu, s);
+    case 2:
+      return fp32 ? launch_sweep<StreamFP32C, 2>(P, p, n_cu, s)
+                  : launch_sweep<StreamFP64, 2>(P, p, n_cu, s);
+    case 4:
+      return fp32 ? launch_sweep<StreamFP32C, 4>(P, p, n_cu, s)
+                  : launch_sweep<StreamFP64, 4>(P, p, n_cu, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -147,7 +155,8 @@ int los_shapes(int fp32, int tp2_size, los_shape *out, int cap)
     for (int p = 0; p < 2 && n < cap; ++p) out[n++] = los_shape{0, NK_, CH_, p};
   LOS_SHAPES(LOS_LIST)
 #undef LOS_LIST
-  for (int nk = 1; nk <= 4 && n < cap; nk *= 2) out[n++] = los_shape{1, nk, 0, 0};
+  for (int nk = 1; nk <= 4; nk *= 2)
+    for (int p = 0; p < 2 && n < cap; ++p) out[n++] = los_shape{1, nk, 0, p};
   for (int b = 1; b <= 4 && n < cap; b *= 2) out[n++] = los_shape{2, 64, b, 0};
   return n;
 }

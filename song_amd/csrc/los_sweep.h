@@ -103,13 +103,18 @@ los_sweep_pairs(const R *src, SweepPair<R> *dst, long rows, int X, int NLP)
 
 // ---------------------------------------------------------------------------
 // Per (fine k3, time step): index_x, a_J and S x dtau of every (m, L, field).
-// One thread per (f, t), t fastest (the tau-contiguous source layout).
+// One thread per (f, t), t fastest (the tau-contiguous source layout). The
+// grid has 8 * per workgroups; workgroup b runs on XCD b % 8 and takes
+// logical block (b % 8) * per + b / 8, so each XCD walks a contiguous eighth
+// of the fine k3 in order and neighbouring k3, which read the same coarse
+// source columns, share that XCD's L2.
 // ---------------------------------------------------------------------------
 template <class R, bool WITH_DT>
 __global__ void __launch_bounds__(256)
-los_sweep_prep(los_params P, int NL)
+los_sweep_prep(los_params P, int NL, int per)
 {
-  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long blk = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const long id = blk * blockDim.x + threadIdx.x;
   if (id >= (long)P.F * P.ntau) return;
   const int f = (int)(id / P.ntau);
   const int t = (int)(id - (long)f * P.ntau);
@@ -130,29 +135,58 @@ los_sweep_prep(los_params P, int NL)
   const double aw = 1 - bw;
   const double ca = aw * aw * aw - aw, cb = bw * bw * bw - bw;
   const double dt = P.dtau[t];
-  auto src = [&](int tp) -> double {
-    if (tp < 0) return 0.0;
-    const long base = pr.src_off + ((long)tp * pr.nc + br) * P.ntau + t;
-    const double *S = P.src + base;
-    double v;
-    if (P.cubic_k3) {
-      const double *D = P.dd + base;
-      v = aw * S[0] + bw * S[P.ntau] + (ca * D[0] + cb * D[P.ntau]) * h * h / 6.0;
-    } else {
-      v = aw * S[0] + bw * S[P.ntau];
-    }
-    return WITH_DT ? v * dt : v;
-  };
+  const long ntau = P.ntau;
+  const double *col = P.src + pr.src_off + (long)br * ntau + t;  // (tp, kc = br, t) at tp = 0
+  const double *dcol = P.dd + pr.src_off + (long)br * ntau + t;
+  const long tstep = (long)pr.nc * ntau;                          // next tp
   SweepS<R> *out = (SweepS<R> *)P.sf + (id * P.m_size) * NL;
-  for (int mi = 0; mi < P.m_size; ++mi)
-    for (int L = 0; L < NL; ++L) {
-      SweepS<R> s;
-      s.T = (R)(P.has_T && L <= P.LmaxT ? src(P.srcT[L][mi]) : 0.0);
-      s.E = (R)(P.has_E && L <= P.LmaxP ? src(P.srcE[L][mi]) : 0.0);
-      s.B = (R)(P.has_B && L <= P.LmaxP ? src(P.srcB[L][mi]) : 0.0);
-      s.pad = 0;
-      out[mi * NL + L] = s;
+  // entries q = mi * NL + L, QB at a time: their 4 x 3 loads per entry are
+  // issued together (absent fields read tp 0 and are replaced by zero)
+  constexpr int QB = 4;
+  const int nq = P.m_size * NL;
+  for (int q0 = 0; q0 < nq; q0 += QB) {
+    int tp[QB][3];
+    double s0[QB][3], s1[QB][3], d0[QB][3], d1[QB][3];
+#pragma unroll
+    for (int u = 0; u < QB; ++u) {
+      const int q = min(q0 + u, nq - 1);
+      const int mi = q / NL, L = q - mi * NL;
+      tp[u][0] = P.has_T && L <= P.LmaxT ? P.srcT[L][mi] : -1;
+      tp[u][1] = P.has_E && L <= P.LmaxP ? P.srcE[L][mi] : -1;
+      tp[u][2] = P.has_B && L <= P.LmaxP ? P.srcB[L][mi] : -1;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const long o = (long)max(tp[u][c], 0) * tstep;
+        s0[u][c] = col[o];
+        s1[u][c] = col[o + ntau];
+        if (P.cubic_k3) {
+          d0[u][c] = dcol[o];
+          d1[u][c] = dcol[o + ntau];
+        }
+      }
     }
+#pragma unroll
+    for (int u = 0; u < QB; ++u) {
+      if (q0 + u >= nq) break;
+      double v[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double w;
+        if (P.cubic_k3)
+          w = aw * s0[u][c] + bw * s1[u][c] + (ca * d0[u][c] + cb * d1[u][c]) * h * h / 6.0;
+        else
+          w = aw * s0[u][c] + bw * s1[u][c];
+        if (WITH_DT) w = w * dt;
+        v[c] = tp[u][c] >= 0 ? w : 0.0;
+      }
+      SweepS<R> o;
+      o.T = (R)v[0];
+      o.E = (R)v[1];
+      o.B = (R)v[2];
+      o.pad = 0;
+      out[q0 + u] = o;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -160,18 +194,15 @@ los_sweep_prep(los_params P, int NL)
 // blockIdx.x = XCD-aware tile index (tiles of XCD b % 8, costliest first).
 // ---------------------------------------------------------------------------
 template <class Pol, int NK, int NL>
-__global__ void __launch_bounds__(64)
-los_sweep(los_params P)
+__device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, int w)
 {
   typedef typename Pol::real R;
   typedef typename Pol::part_t Q;
   typedef typename Pol::acc_t A;
   constexpr int FOLD = 32;  // time steps per partial sum
 
-  const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
   if (q >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return;
   const int tile = P.xcd_tile[xcd + 1] - 1 - q;
-  const int w = blockIdx.y;
   const int nchunk = P.n_lp >> 6;
   const int mi = w / nchunk;
   const int m = P.m_of_mi[mi];
@@ -297,35 +328,98 @@ los_sweep(los_params P)
   }
 }
 
+// Item order shared by both sweep kernels. Dispatch form (PERS false): one
+// workgroup per item, grid (8 x tiles of the largest XCD range, waves per
+// tile). Persistent form (PERS true): the grid is the number of waves the GPU
+// holds at once; wave p of an XCD takes that XCD's items p, p + stride, ...
+// (item i = wave class i / ntile, tile position i % ntile), so the waves
+// resident together work on neighbouring k3 of one class and advance through
+// tau together, and the J rows they read stay in the XCD's L2.
+#define LOS_SWEEP_ITEMS(CALL)                                   \
+  const int xcd = blockIdx.x & 7;                               \
+  if (!PERS) {                                                  \
+    CALL(P, xcd, (int)(blockIdx.x >> 3), (int)blockIdx.y);      \
+    return;                                                     \
+  }                                                             \
+  const int ntile = P.xcd_tile[xcd + 1] - P.xcd_tile[xcd];      \
+  const int stride = (int)(gridDim.x >> 3);                     \
+  const int nitems = ntile * P.n_wave;                          \
+  for (int i = (int)(blockIdx.x >> 3); i < nitems; i += stride) { \
+    const int w = i / ntile;                                    \
+    CALL(P, xcd, i - w * ntile, w);                             \
+  }
+
+// Sweep kernel, one wave per workgroup (see sweep_item).
+template <class Pol, int NK, int NL, bool PERS>
+__global__ void __launch_bounds__(64)
+los_sweep(los_params P)
+{
+#define LOS_CALL_FAST(...) sweep_item<Pol, NK, NL>(__VA_ARGS__)
+  LOS_SWEEP_ITEMS(LOS_CALL_FAST)
+#undef LOS_CALL_FAST
+}
+
 template <class R, bool WITH_DT>
 static hipError_t launch_sweep_prep(const los_params *P, hipStream_t s)
 {
   const long n = (long)P->F * P->ntau;
   if (n == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     s, *P, NL);
+  const long nb = (n + 255) / 256, per = (nb + 7) / 8;
+  hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT>), dim3((unsigned)(8 * per)), dim3(256), 0, s, *P,
+                     NL, (int)per);
   return hipGetLastError();
 }
 
-// P->tiles / P->xcd_tile must describe tiles of at most NK positions.
-template <class Pol, int NK>
-static hipError_t launch_sweep(const los_params *P, hipStream_t s)
+// Grid of a sweep launch: dispatch form (8 x largest XCD tile range, waves
+// per tile) or persistent form (8 x the waves each XCD holds at once of kernel
+// k, from the occupancy of its 64-thread workgroups on n_cu CUs).
+static hipError_t sweep_grid(const los_params *P, bool pers, const void *k, int n_cu, dim3 *grid)
 {
-  if (P->n_pos == 0 || P->n_wave == 0) return hipSuccess;
   int most = 0;
   for (int x = 0; x < 8; ++x) {
     const int n = P->xcd_tile[x + 1] - P->xcd_tile[x];
     most = n > most ? n : most;
   }
-  if (most == 0) return hipSuccess;
-  const dim3 grid(8 * most, P->n_wave);
+  if (most == 0 || P->n_wave == 0) {
+    *grid = dim3(0, 0);
+    return hipSuccess;
+  }
+  if (!pers) {
+    *grid = dim3(8 * most, P->n_wave);
+    return hipSuccess;
+  }
+  int per_cu = 0;
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0);
+  if (e != hipSuccess) return e;
+  if (per_cu < 1) per_cu = 1;
+  const long waves = ((long)n_cu * per_cu + 7) / 8;                    // per XCD
+  const long items = (long)most * P->n_wave;
+  *grid = dim3((unsigned)(8 * (waves < items ? waves : items)), 1);
+  return hipSuccess;
+}
+
+// P->tiles / P->xcd_tile must describe tiles of at most NK positions.
+template <class Pol, int NK>
+static hipError_t launch_sweep(const los_params *P, bool pers, int n_cu, hipStream_t s)
+{
+  if (P->n_pos == 0 || P->n_wave == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  switch (NL) {
-#define LOS_SW(n) \
-  case n: hipLaunchKernelGGL((los_sweep<Pol, NK, n>), grid, dim3(64), 0, s, *P); break;
+  switch (NL * 2 + (pers ? 1 : 0)) {
+#define LOS_SW1(n, b)                                                                 \
+  case n * 2 + b: {                                                                   \
+    dim3 grid;                                                                        \
+    const hipError_t e =                                                              \
+        sweep_grid(P, b, (const void *)los_sweep<Pol, NK, n, b>, n_cu, &grid);        \
+    if (e != hipSuccess) return e;                                                    \
+    if (grid.x == 0) return hipSuccess;                                               \
+    hipLaunchKernelGGL((los_sweep<Pol, NK, n, b>), grid, dim3(64), 0, s, *P);         \
+    break;                                                                            \
+  }
+#define LOS_SW(n) LOS_SW1(n, false) LOS_SW1(n, true)
     LOS_SW(1) LOS_SW(2) LOS_SW(3) LOS_SW(4) LOS_SW(5) LOS_SW(6) LOS_SW(7) LOS_SW(8)
 #undef LOS_SW
+#undef LOS_SW1
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -342,13 +436,10 @@ static hipError_t launch_sweep(const los_params *P, hipStream_t s)
 // (source/transfer2.c:2259-2329, :1861-2098).
 // ---------------------------------------------------------------------------
 template <int NK, int NL>
-__global__ void __launch_bounds__(64)
-los_sweep_exact(los_params P)
+__device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, int q, int w)
 {
-  const int xcd = blockIdx.x & 7, q = blockIdx.x >> 3;
   if (q >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return;
   const int tile = P.xcd_tile[xcd + 1] - 1 - q;
-  const int w = blockIdx.y;
   const int nchunk = P.n_lp >> 6;
   const int mi = w / nchunk;
   const int m = P.m_of_mi[mi];
@@ -497,23 +588,35 @@ los_sweep_exact(los_params P)
   }
 }
 
+template <int NK, int NL, bool PERS>
+__global__ void __launch_bounds__(64)
+los_sweep_exact(los_params P)
+{
+#define LOS_CALL_EXACT(...) sweep_exact_item<NK, NL>(__VA_ARGS__)
+  LOS_SWEEP_ITEMS(LOS_CALL_EXACT)
+#undef LOS_CALL_EXACT
+}
+
 template <int NK>
-static hipError_t launch_sweep_exact(const los_params *P, hipStream_t s)
+static hipError_t launch_sweep_exact(const los_params *P, bool pers, int n_cu, hipStream_t s)
 {
   if (P->n_pos == 0 || P->n_wave == 0) return hipSuccess;
-  int most = 0;
-  for (int x = 0; x < 8; ++x) {
-    const int n = P->xcd_tile[x + 1] - P->xcd_tile[x];
-    most = n > most ? n : most;
-  }
-  if (most == 0) return hipSuccess;
-  const dim3 grid(8 * most, P->n_wave);
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  switch (NL) {
-#define LOS_SWX(n) \
-  case n: hipLaunchKernelGGL((los_sweep_exact<NK, n>), grid, dim3(64), 0, s, *P); break;
+  switch (NL * 2 + (pers ? 1 : 0)) {
+#define LOS_SWX1(n, b)                                                                \
+  case n * 2 + b: {                                                                   \
+    dim3 grid;                                                                        \
+    const hipError_t e =                                                              \
+        sweep_grid(P, b, (const void *)los_sweep_exact<NK, n, b>, n_cu, &grid);       \
+    if (e != hipSuccess) return e;                                                    \
+    if (grid.x == 0) return hipSuccess;                                               \
+    hipLaunchKernelGGL((los_sweep_exact<NK, n, b>), grid, dim3(64), 0, s, *P);        \
+    break;                                                                            \
+  }
+#define LOS_SWX(n) LOS_SWX1(n, false) LOS_SWX1(n, true)
     LOS_SWX(1) LOS_SWX(2) LOS_SWX(3) LOS_SWX(4) LOS_SWX(5) LOS_SWX(6) LOS_SWX(7) LOS_SWX(8)
 #undef LOS_SWX
+#undef LOS_SWX1
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -15,7 +15,10 @@ from tests.conftest import field_errors, run_oracle
 from tests.test_gpu_parity import CASES
 
 SHAPES = ([(0, nk, ch, p) for nk in (2, 4, 8) for ch in (16, 32) for p in (0, 1)] +
-          [(1, nk, 0, 0) for nk in (1, 2, 4)] + [(2, 64, b, 0) for b in (1, 2, 4)])
+          [(1, nk, 0, p) for nk in (1, 2, 4) for p in (0, 1)] + [(2, 64, b, 0) for b in (1, 2, 4)])
+# the plain form's kernels: per-k3 workgroups, sweep (dispatch and persistent), k-lane
+FP64_SHAPES = ([(0, 1, 0, 0)] + [(1, nk, 0, p) for nk in (1, 2, 4) for p in (0, 1)] +
+               [(2, 64, b, 0) for b in (1, 2, 4)])
 
 # log k in [1e-5, 0.2], 8 coarse k3 per pair, fine step 5e-4: fine k3 counts from 8
 # (narrow triangles, coarse grid only) to about 400 (k1 = k2 = 0.2)
@@ -44,7 +47,7 @@ def test_every_tile_shape(oracle):
     eng = L.Engine(0)
     eng.load(s)
     eng.set_form(L.FORM_FP64)
-    for sh in ((0, 1, 0), (1, 1, 0), (1, 2, 0), (1, 4, 0), (2, 64, 1), (2, 64, 2), (2, 64, 4)):
+    for sh in FP64_SHAPES:
         # every kernel of the plain form is bit-exact
         eng.set_shape(L.FORM_FP64, *sh)
         print(f"form 0 shape {sh}", flush=True)
@@ -99,7 +102,7 @@ def test_uneven_k3_counts(oracle):
     eng = L.Engine(0)
     eng.load(s)
     eng.set_form(L.FORM_FP64)
-    for sh in ((0, 1, 0), (1, 2, 0), (2, 64, 2)):
+    for sh in ((0, 1, 0), (1, 2, 0), (1, 2, 0, 1), (2, 64, 2)):
         eng.set_shape(L.FORM_FP64, *sh)
         assert np.array_equal(_run(eng, s, L.FORM_FP64), ref), f"fp64 shape {sh}"
     tt2 = _tt2(s, ref)
@@ -109,7 +112,7 @@ def test_uneven_k3_counts(oracle):
         print(f"form {form} tuned shape {eng.get_shape(form)}", flush=True)
         err = field_errors(_run(eng, s, form), ref, tt2)
         assert err.max() < tol, f"form {form}: {err.max():.3e}"
-        for sh in ((0, 4, 32, 1), (1, 2, 0, 0), (2, 64, 4, 0)):  # work queue; sweep; k-lane
+        for sh in ((0, 4, 32, 1), (1, 2, 0, 0), (1, 4, 0, 1), (2, 64, 4, 0)):
             eng.set_shape(form, *sh)
             err = field_errors(_run(eng, s, form), ref, tt2)
             assert err.max() < tol, f"form {form} shape {sh}: {err.max():.3e}"
@@ -119,7 +122,7 @@ def test_uneven_k3_counts(oracle):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("form,shape", [(L.FORM_FP64, (0, 1)), (L.FORM_FP64, (2, 64, 2)),
-                                        (L.FORM_FP64, (1, 2)),
+                                        (L.FORM_FP64, (1, 2)), (L.FORM_FP64, (1, 2, 0, 1)),
                                         (L.FORM_FP32C, (0, 4, 32, 1)),
                                         (L.FORM_FP64_FAST, (0, 8, 16, 0)),
                                         (L.FORM_FP32C, (1, 2, 0, 0)),
