@@ -300,6 +300,8 @@ def main(argv=None):
                     help="replay each step as one captured hipGraph (auto: batched workloads)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-jbuild", action="store_true",
+                    help="skip timing the J-table builds (host and GPU)")
     ap.add_argument("--errors-out", default="", help="write per-(l,m) errors as JSON here")
     ap.add_argument("--copy-probe-mib", type=float, default=0.0,
                     help="run a plain device copy of this size once (HBM counter calibration)")
@@ -320,8 +322,20 @@ def main(argv=None):
         return serve_bench(args, cfg)
     t0 = time.time()
     scen = L.Scenario(cfg)
-    scen.prepare()
     print(f"# inputs built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    jtab = None
+    if not args.no_jbuild:
+        # the workload's J tables built again on the GPU (twice: the first call includes the
+        # device's start-up) and on the host, which leaves the host-built tables in place
+        scen.rebuild_J(gpu=True)
+        t_gpu = scen.rebuild_J(gpu=True)
+        t_host = scen.rebuild_J(gpu=False)
+        jtab = {"gpu_s": t_gpu, "host_s": t_host,
+                "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                "x_points": int(cfg.xx_max / cfg.xx_step + 1), "l_max": cfg.l_max}
+        print(f"# J tables: GPU build {t_gpu:.3f} s, host build {t_host:.3f} s", file=sys.stderr,
+              flush=True)
+    scen.prepare()
     all_pairs = [(i1, i2) for i1 in range(cfg.k_size) for i2 in range(i1 + 1)]
     pairs = select_pairs(cfg, PAIRS[args.workload]) if args.workload in PAIRS else all_pairs
     eng = L.Engine(0)
@@ -432,6 +446,8 @@ def main(argv=None):
     }
     if probe:
         line["copy_probe"] = probe
+    if jtab:
+        line["j_tables"] = jtab
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(cfg, pairs, args.cpu_seconds)
     print(json.dumps(line), flush=True)

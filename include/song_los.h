@@ -11,8 +11,9 @@
  *  - los_scenario_* : build seeded synthetic inputs (k grids, source tables,
  *                     J_Llm(x) tables) in SONG's structures, for tests,
  *                     benchmarks and users without a perturbations2 run.
- *  - los_bessel2_build : FP64 builder of struct bessels2 (the J_Llm(x) tables
- *                     that source/bessel2.c:75-468 produces in SONG).
+ *  - los_bessel2_build, los_bessel2_build_gpu : FP64 builders of struct
+ *                     bessels2 (the J_Llm(x) tables that source/bessel2.c:75-468
+ *                     produces in SONG), on the host or on the GPU.
  *
  * All functions return _SUCCESS_ (0) or _FAILURE_ (1) and write a message into
  * the caller's ErrorMsg buffer on failure.
@@ -221,6 +222,8 @@ typedef struct los_config {
   double tau_peak, tau_sigma, ramp_amp, ramp_center, ramp_width;
   /* verbosity of the stage */
   int verbose;
+  /* J tables: built on the host (0) or on the GPU (1, default device) */
+  int j_build;
 } los_config;
 
 /* Defaults: a small plain FP64 temperature-only configuration (m = 0, L = 0..4, 4x4 pairs). */
@@ -263,6 +266,10 @@ int los_scenario_k3_sizes(los_scenario *s, int *out, long n, char *errmsg);
 double *los_scenario_source_ptr(los_scenario *s, int index_tp, int index_k1, int index_k2);
 double *los_scenario_J_row(los_scenario *s, int index_J, int index_L, int index_l, int index_m,
                            int *xmin, int *xsize);
+/* Free the scenario's J tables and build them again on the host (gpu = 0) or on
+   HIP device `device` (gpu = 1); *seconds receives the wall time of the build. */
+int los_scenario_rebuild_J(los_scenario *s, int gpu, int device, double *seconds,
+                           char *errmsg);
 
 /* FP64 builder of the projection-function tables of struct bessels2, with the
    semantics of source/bessel2.c:75-468 (xx grid, index_xmin_J by bisection on
@@ -270,6 +277,15 @@ double *los_scenario_J_row(los_scenario *s, int index_J, int index_L, int index_
 int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
                       struct bessels *pbs, struct bessels2 *pbs2, char *errmsg);
 int los_bessel2_free(struct bessels *pbs, struct bessels2 *pbs2);
+
+/* The same tables built on HIP device `device`: the j_l(x) table of every grid
+   point, the index_xmin_J bisections and the row fills run on the GPU
+   (source/bessel2.c:1041-1588, :1669-1815 semantics); the rows land in the
+   host arrays of struct bessels2 as with los_bessel2_build. No counterpart
+   in SONG, whose bessel2 module builds them on the host. */
+int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                          struct bessels *pbs, struct bessels2 *pbs2, int device,
+                          char *errmsg);
 
 /* Special functions used by the builder (exported for tests). */
 void los_sph_bessel_array(int lmax, double x, double *out);

@@ -27,6 +27,7 @@ HIP_SOURCES = [
     ("los_kernels_exact.hip", ["-ffp-contract=off"]),
     ("los_kernels_fast.hip", []),
     ("los_serve.hip", []),
+    ("los_bessel2_gpu.hip", ["-ffp-contract=off"]),
 ]
 
 C_FLAGS = ["-O2", "-std=c99", "-D_GNU_SOURCE", "-fPIC", "-fopenmp", "-Wall",

@@ -27,13 +27,7 @@
 #include <omp.h>
 #endif
 
-typedef struct {
-  int J, iL, il, im; /* indices */
-  int nterm;
-  int l1[32];
-  double coef[32];
-  int xmin;
-} jrow;
+typedef los_jrow jrow;
 
 static int sign_of_power(int n) { return (n % 2 == 0) ? 1 : -1; }
 
@@ -49,8 +43,12 @@ static double row_value(const jrow *r, const double *jl, const int *xmin_l1, int
   return acc;
 }
 
-int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
-                      struct bessels *pbs, struct bessels2 *pbs2, char *errmsg)
+/* Shared first phase of both builders: the x grid, the J types, the [J][L][l][m]
+   index arrays and the rows with their l1 term lists (3j coefficients). Rows
+   whose J vanishes identically get xmin = xx_size - 1, the others xmin = -1. */
+int los_bessel2_setup(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                      struct bessels *pbs, struct bessels2 *pbs2, los_jrow **rows_out,
+                      int *nrows_out, int *l1max_out, char *errmsg)
 {
   /* xx grid */
   pbs2->xx_step = ppr2->bessel_x_step_song;
@@ -88,33 +86,7 @@ int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct per
 
   const int l_size = pbs->l_size;
   const int l1max = pbs->l[l_size - 1] + pbs2->L_max + 2;
-
-  /* first grid index where |j_l1| >= j_cut, minus one (the stored range
-     starts at the point preceding the threshold) */
-  int *xmin_l1;
-  los_alloc(xmin_l1, (l1max + 1) * sizeof(int), errmsg);
-  {
-    double *jl = malloc((l1max + 1) * sizeof(double));
-    los_test(jl == NULL, errmsg, "out of memory");
-    /* j_l1 grows monotonically for x < l1; bisection over grid indices per l1 */
-    for (int l1 = 0; l1 <= l1max; ++l1) {
-      if (l1 == 0) { xmin_l1[0] = 0; continue; }
-      int hi = 0;
-      double xpeak = l1 + 0.5;
-      if (xpeak >= pbs2->xx[pbs2->xx_size - 1]) hi = pbs2->xx_size - 1;
-      else { hi = (int)(xpeak / pbs2->xx_step); if (hi >= pbs2->xx_size) hi = pbs2->xx_size - 1; }
-      los_sph_bessel_array(l1, pbs2->xx[hi], jl);
-      if (fabs(jl[l1]) < pbs2->j_l1_cut) { xmin_l1[l1] = pbs2->xx_size - 1; continue; }
-      int lo = 0;
-      while (hi - lo > 1) {
-        int mid = (hi + lo) / 2;
-        los_sph_bessel_array(l1, pbs2->xx[mid], jl);
-        if (fabs(jl[l1]) >= pbs2->j_l1_cut) hi = mid; else lo = mid;
-      }
-      xmin_l1[l1] = lo;
-    }
-    free(jl);
-  }
+  *l1max_out = l1max;
 
   /* Enumerate rows and their l1 coefficient lists. */
   int nrows = 0;
@@ -205,6 +177,91 @@ int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct per
       }
     }
   }
+  *rows_out = rows;
+  *nrows_out = nrows;
+  return _SUCCESS_;
+}
+
+/* Row arrays of struct bessels2 once every row's xmin is known: index_xmin_J,
+   x_size_J, x_min_J and zeroed J (and ddJ) rows from xmin to the grid end. */
+int los_bessel2_alloc_rows(struct bessels2 *pbs2, const los_jrow *rows, int nrows, char *errmsg)
+{
+  pbs2->count_allocated_Js = 0;
+  pbs2->x_size_max_J = 0;
+  for (int q = 0; q < nrows; ++q) {
+    const jrow *row = &rows[q];
+    int xs = pbs2->xx_size - row->xmin;
+    pbs2->index_xmin_J[row->J][row->iL][row->il][row->im] = row->xmin;
+    pbs2->x_size_J[row->J][row->iL][row->il][row->im] = xs;
+    pbs2->x_min_J[row->J][row->iL][row->il][row->im] =
+        (row->xmin == pbs2->xx_size - 1) ? pbs2->xx_max : pbs2->xx[row->xmin];
+    double *p;
+    los_calloc(p, xs, sizeof(double), errmsg);
+    pbs2->J_Llm_x[row->J][row->iL][row->il][row->im] = p;
+    if (pbs2->ddJ_Llm_x) {
+      double *d;
+      los_calloc(d, xs, sizeof(double), errmsg);
+      pbs2->ddJ_Llm_x[row->J][row->iL][row->il][row->im] = d;
+    }
+    pbs2->count_allocated_Js += xs;
+    if (xs > pbs2->x_size_max_J) pbs2->x_size_max_J = xs;
+  }
+  return _SUCCESS_;
+}
+
+/* Second derivatives of the J rows for cubic interpolation in x: estimated end
+   derivatives for l == 2, natural ends otherwise (source/bessel2.c:395-463). */
+void los_bessel2_spline_rows(struct bessels *pbs, struct bessels2 *pbs2, const los_jrow *rows,
+                             int nrows)
+{
+  if (!pbs2->ddJ_Llm_x) return;
+  double *work = malloc(pbs2->xx_size * sizeof(double));
+  for (int q = 0; q < nrows; ++q) {
+    const jrow *row = &rows[q];
+    int xs = pbs2->x_size_J[row->J][row->iL][row->il][row->im];
+    if (xs < 3) continue;
+    int natural = (pbs->l[row->il] == 2) ? 0 : 1;
+    los_spline_one(pbs2->xx + row->xmin, xs, pbs2->J_Llm_x[row->J][row->iL][row->il][row->im],
+                   pbs2->ddJ_Llm_x[row->J][row->iL][row->il][row->im], natural, work);
+  }
+  free(work);
+}
+
+int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                      struct bessels *pbs, struct bessels2 *pbs2, char *errmsg)
+{
+  jrow *rows;
+  int nrows, l1max;
+  if (los_bessel2_setup(ppr, ppr2, ppt2, pbs, pbs2, &rows, &nrows, &l1max, errmsg) == _FAILURE_)
+    return _FAILURE_;
+
+  /* first grid index where |j_l1| >= j_cut, minus one (the stored range
+     starts at the point preceding the threshold) */
+  int *xmin_l1;
+  los_alloc(xmin_l1, (l1max + 1) * sizeof(int), errmsg);
+  {
+    double *jl = malloc((l1max + 1) * sizeof(double));
+    los_test(jl == NULL, errmsg, "out of memory");
+    /* j_l1 grows monotonically for x < l1; bisection over grid indices per l1 */
+    for (int l1 = 0; l1 <= l1max; ++l1) {
+      if (l1 == 0) { xmin_l1[0] = 0; continue; }
+      int hi = 0;
+      double xpeak = l1 + 0.5;
+      if (xpeak >= pbs2->xx[pbs2->xx_size - 1]) hi = pbs2->xx_size - 1;
+      else { hi = (int)(xpeak / pbs2->xx_step); if (hi >= pbs2->xx_size) hi = pbs2->xx_size - 1; }
+      los_sph_bessel_array(l1, pbs2->xx[hi], jl);
+      if (fabs(jl[l1]) < pbs2->j_l1_cut) { xmin_l1[l1] = pbs2->xx_size - 1; continue; }
+      int lo = 0;
+      while (hi - lo > 1) {
+        int mid = (hi + lo) / 2;
+        los_sph_bessel_array(l1, pbs2->xx[mid], jl);
+        if (fabs(jl[l1]) >= pbs2->j_l1_cut) hi = mid; else lo = mid;
+      }
+      xmin_l1[l1] = lo;
+    }
+    free(jl);
+  }
+
 
   /* index_xmin_J by bisection on the row values. */
   int abort_flag = 0;
@@ -238,27 +295,7 @@ int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct per
   }
   if (abort_flag) return _FAILURE_;
 
-  /* allocate rows */
-  pbs2->count_allocated_Js = 0;
-  pbs2->x_size_max_J = 0;
-  for (int q = 0; q < nrows; ++q) {
-    jrow *row = &rows[q];
-    int xs = pbs2->xx_size - row->xmin;
-    pbs2->index_xmin_J[row->J][row->iL][row->il][row->im] = row->xmin;
-    pbs2->x_size_J[row->J][row->iL][row->il][row->im] = xs;
-    pbs2->x_min_J[row->J][row->iL][row->il][row->im] =
-        (row->xmin == pbs2->xx_size - 1) ? pbs2->xx_max : pbs2->xx[row->xmin];
-    double *p;
-    los_calloc(p, xs, sizeof(double), errmsg);
-    pbs2->J_Llm_x[row->J][row->iL][row->il][row->im] = p;
-    if (pbs2->ddJ_Llm_x) {
-      double *d;
-      los_calloc(d, xs, sizeof(double), errmsg);
-      pbs2->ddJ_Llm_x[row->J][row->iL][row->il][row->im] = d;
-    }
-    pbs2->count_allocated_Js += xs;
-    if (xs > pbs2->x_size_max_J) pbs2->x_size_max_J = xs;
-  }
+  if (los_bessel2_alloc_rows(pbs2, rows, nrows, errmsg) == _FAILURE_) return _FAILURE_;
 
   /* Fill rows: sweep x, one j_l array per x. */
   const int xs_total = pbs2->xx_size;
@@ -283,19 +320,7 @@ int los_bessel2_build(struct precision *ppr, struct precision2 *ppr2, struct per
     free(jl);
   }
 
-  /* cubic interpolation of J: second derivatives per row */
-  if (pbs2->ddJ_Llm_x) {
-    double *work = malloc(pbs2->xx_size * sizeof(double));
-    for (int q = 0; q < nrows; ++q) {
-      jrow *row = &rows[q];
-      int xs = pbs2->x_size_J[row->J][row->iL][row->il][row->im];
-      if (xs < 3) continue;
-      int natural = (pbs->l[row->il] == 2) ? 0 : 1;
-      los_spline_one(pbs2->xx + row->xmin, xs, pbs2->J_Llm_x[row->J][row->iL][row->il][row->im],
-                     pbs2->ddJ_Llm_x[row->J][row->iL][row->il][row->im], natural, work);
-    }
-    free(work);
-  }
+  los_bessel2_spline_rows(pbs, pbs2, rows, nrows);
 
   free(rows);
   free(xmin_l1);

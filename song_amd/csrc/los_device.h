@@ -51,6 +51,7 @@ typedef struct {
 typedef struct {
   /* pairs and fine k3 */
   const los_pair *pairs;
+  int n_pairs;
   const double *kc;     /* coarse k3, concatenated per pair */
   const double *kf;     /* fine k3, concatenated per pair (F values) */
   const int *fpair;     /* pair of each fine k3 */
@@ -120,7 +121,7 @@ typedef struct {
   int m_of_mi[LOS_MAX_MI]; /* m value of each m-list entry */
   int n_wave;            /* waves per tile: m_size * n_lp / 64 */
   void *ixa;             /* per (f, t): index_x and a_J ({int, float} or {int, pad, double}) */
-  void *sf;              /* per (f, t, mi, L): S x trapezoid weight of T, E, B (+ pad) */
+  void *sf;              /* per (f, t, mi): [T, E, B][L] sources (x trapezoid weight: fast forms) */
   const int *tau_end_fw; /* per (f, wave): time steps before every row of the wave is dead */
 
   /* k-lane kernels (one (l,m) per wave, one fine k3 per lane): tables by rank

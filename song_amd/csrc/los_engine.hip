@@ -581,6 +581,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   los_params &P = e->P;
   memset(&P, 0, sizeof P);
   P.pairs = e->dpairs;
+  P.n_pairs = (int)e->pairs.size();
   P.kc = e->dkc;
   P.kf = e->dkf;
   P.fpair = e->dfpair;
@@ -851,7 +852,7 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
   const size_t n = (size_t)e->F * e->ntau;
   const size_t rb = fp32 ? 4 : 8;
   const size_t ixa_b = std::max<size_t>(1, n) * (fp32 ? 8 : 16);
-  const size_t sf_b = std::max<size_t>(1, n * e->m_size * NL) * 4 * rb;
+  const size_t sf_b = std::max<size_t>(1, n * e->m_size * NL) * 3 * rb;
   HIPCHK(hipMalloc(&e->dixa, ixa_b), err);
   HIPCHK(hipMalloc(&e->dsf, sf_b), err);
   e->P.ixa = e->dixa;

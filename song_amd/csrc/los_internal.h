@@ -73,6 +73,27 @@ double los_engine_batch_budget(los_engine *eng);
 double los_engine_batch_bytes(struct precision2 *ppr2, struct perturbs2 *ppt2,
                               struct transfers2 *ptr2, int k1_begin, int k1_end);
 
+/* ---- J-table builders (los_bessel2.c, los_bessel2_gpu.hip) ---- */
+/* One J_Llm row: its indices in [J][L][l][m], the l1 terms of
+   sum_l1 coef * j_l1(x), and the first stored x index. */
+typedef struct {
+  int J, iL, il, im;
+  int nterm;
+  int l1[32];
+  double coef[32];
+  int xmin;
+} los_jrow;
+int los_bessel2_setup(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                      struct bessels *pbs, struct bessels2 *pbs2, los_jrow **rows,
+                      int *nrows, int *l1max, char *errmsg);
+int los_bessel2_alloc_rows(struct bessels2 *pbs2, const los_jrow *rows, int nrows,
+                           char *errmsg);
+void los_bessel2_spline_rows(struct bessels *pbs, struct bessels2 *pbs2, const los_jrow *rows,
+                             int nrows);
+int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                          struct bessels *pbs, struct bessels2 *pbs2, int device,
+                          char *errmsg);
+
 /* ---- special functions (los_special.c) ---- */
 void los_sph_bessel_array(int lmax, double x, double *out);
 int los_wigner3j_j1(int j2, int j3, int m2, int m3, int *j1min, int *j1max,

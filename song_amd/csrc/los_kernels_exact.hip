@@ -55,7 +55,7 @@ los_relayout_spline(const los_pair *pairs, const double *kc_all, const double *s
   // rows t0 .. t0+nt-1 of [tau][kc] are contiguous: nt * n doubles, read in
   // batches of LOS_RS_B loads per lane so that they are all in flight at once
   {
-    constexpr int LOS_RS_B = 16;
+    constexpr int LOS_RS_B = 32;
     const double *r = raw + base + (long)t0 * n;
     const int total = nt * n;
     for (int i0 = 0; i0 < total; i0 += LOS_RS_TW * LOS_RS_B) {

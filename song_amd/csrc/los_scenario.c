@@ -18,6 +18,7 @@
 #include "los_internal.h"
 
 #include <math.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -355,7 +356,11 @@ int los_scenario_create(const los_config *cfg, los_scenario **out, char *err)
   /* ---------------- bessels ---------------- */
   pbs->x_max = c->xx_max;
   pbs2->xx_max = c->xx_max;
-  if (los_bessel2_build(ppr, ppr2, ppt2, pbs, pbs2, err) == _FAILURE_) return _FAILURE_;
+  if (c->j_build) {
+    if (los_bessel2_build_gpu(ppr, ppr2, ppt2, pbs, pbs2, 0, err) == _FAILURE_) return _FAILURE_;
+  } else if (los_bessel2_build(ppr, ppr2, ppt2, pbs, pbs2, err) == _FAILURE_) {
+    return _FAILURE_;
+  }
 
   /* ---------------- transfers2 settings ---------------- */
   ptr2->k_sampling = c->k3_sampling;
@@ -466,6 +471,18 @@ double *los_scenario_J_row(los_scenario *s, int J, int L, int il, int im, int *x
   *xmin = b->index_xmin_J[J][L][il][im];
   *xsize = b->x_size_J[J][L][il][im];
   return b->J_Llm_x[J][L][il][im];
+}
+
+int los_scenario_rebuild_J(los_scenario *s, int gpu, int device, double *seconds, char *err)
+{
+  struct timespec t0, t1;
+  los_bessel2_free(&s->bs, &s->bs2);
+  clock_gettime(CLOCK_MONOTONIC, &t0);
+  int rc = gpu ? los_bessel2_build_gpu(&s->pr, &s->pr2, &s->pt2, &s->bs, &s->bs2, device, err)
+               : los_bessel2_build(&s->pr, &s->pr2, &s->pt2, &s->bs, &s->bs2, err);
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  if (seconds) *seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+  return rc;
 }
 
 /* Copies of the generated grids (l list, k grid, tau sampling). */

@@ -34,10 +34,6 @@ struct SweepIXA<double> {
   double a;
 };
 
-template <class R>
-struct SweepS {
-  R T, E, B, pad;
-};
 
 // Both interpolation nodes of a J row at one x slot, J[x] and J[x+1], side by
 // side so that a lane reads them with one load.
@@ -102,89 +98,99 @@ los_sweep_pairs(const R *src, SweepPair<R> *dst, long rows, int X, int NLP)
 }
 
 // ---------------------------------------------------------------------------
-// Per (fine k3, time step): index_x, a_J and S x dtau of every (m, L, field).
-// One thread per (f, t), t fastest (the tau-contiguous source layout). The
-// grid has 8 * per workgroups; workgroup b runs on XCD b % 8 and takes
-// logical block (b % 8) * per + b / 8, so each XCD walks a contiguous eighth
-// of the fine k3 in order and neighbouring k3, which read the same coarse
-// source columns, share that XCD's L2.
+// Per (fine k3, time step) tables of the sweep kernels: index_x and a_J
+// (source/transfer2.c:2195-2202) and, per (m, field), the sources of every L
+// interpolated in k3 (source/transfer2.c:2700-2706), times the trapezoid
+// weight (source/transfer2.c:2317) in the fast forms:
+//   ixa[f][t] = {index_x, a_J},   sf[f][t][mi][field][L] (field T, E, B).
+// One thread per (pair, mi, field, t), t fastest: it walks the pair's fine k3
+// in order and keeps the two coarse columns of the current bracket (sources
+// and their spline second derivatives, for every L) in registers; moving to
+// the next bracket loads only the new right-hand column, so every coarse
+// source column is read once. Workgroup b runs on XCD b % 8 and takes logical
+// block (b % 8) * per + b / 8 (each XCD a contiguous range of pairs).
 // ---------------------------------------------------------------------------
-template <class R, bool WITH_DT>
+template <class R, bool WITH_DT, int NL>
 __global__ void __launch_bounds__(256)
-los_sweep_prep(los_params P, int NL, int per)
+los_sweep_prep(los_params P, int per)
 {
   const long blk = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
   const long id = blk * blockDim.x + threadIdx.x;
-  if (id >= (long)P.F * P.ntau) return;
-  const int f = (int)(id / P.ntau);
-  const int t = (int)(id - (long)f * P.ntau);
-  const double k = P.kf[f];
-  const double x = k * P.t0mt[t];
-  const int ix = (int)(x / P.xx_step);
-  SweepIXA<R> o;
-  o.ix = ix;
-  o.a = (R)((P.xx[ix + 1] - x) / P.xx_step);
-  ((SweepIXA<R> *)P.ixa)[id] = o;
+  const long nt = P.ntau;
+  const long per_pair = (long)P.m_size * 3 * nt;
+  if (id >= (long)P.n_pairs * per_pair) return;
+  const int p = (int)(id / per_pair);
+  const long r = id - (long)p * per_pair;
+  const int mc = (int)(r / nt);
+  const int t = (int)(r - (long)mc * nt);
+  const int mi = mc / 3, c = mc - mi * 3;
+  const los_pair pr = P.pairs[p];
 
-  const los_pair pr = P.pairs[P.fpair[f]];
-  const int br = P.fbr[f];
-  const double *kc = P.kc + pr.kc_off;
-  const double ks = P.kf[P.fsrc[f]];
-  const double h = kc[br + 1] - kc[br];
-  const double bw = (ks - kc[br]) / h;
-  const double aw = 1 - bw;
-  const double ca = aw * aw * aw - aw, cb = bw * bw * bw - bw;
-  const double dt = P.dtau[t];
-  const long ntau = P.ntau;
-  const double *col = P.src + pr.src_off + (long)br * ntau + t;  // (tp, kc = br, t) at tp = 0
-  const double *dcol = P.dd + pr.src_off + (long)br * ntau + t;
-  const long tstep = (long)pr.nc * ntau;                          // next tp
-  SweepS<R> *out = (SweepS<R> *)P.sf + (id * P.m_size) * NL;
-  // entries q = mi * NL + L, QB at a time: their 4 x 3 loads per entry are
-  // issued together (absent fields read tp 0 and are replaced by zero)
-  constexpr int QB = 4;
-  const int nq = P.m_size * NL;
-  for (int q0 = 0; q0 < nq; q0 += QB) {
-    int tp[QB][3];
-    double s0[QB][3], s1[QB][3], d0[QB][3], d1[QB][3];
+  // source type of each L for this (m, field); absent ones read type 0 and give 0
+  int tp[NL];
 #pragma unroll
-    for (int u = 0; u < QB; ++u) {
-      const int q = min(q0 + u, nq - 1);
-      const int mi = q / NL, L = q - mi * NL;
-      tp[u][0] = P.has_T && L <= P.LmaxT ? P.srcT[L][mi] : -1;
-      tp[u][1] = P.has_E && L <= P.LmaxP ? P.srcE[L][mi] : -1;
-      tp[u][2] = P.has_B && L <= P.LmaxP ? P.srcB[L][mi] : -1;
+  for (int L = 0; L < NL; ++L) {
+    int v = -1;
+    if (c == 0 && P.has_T && L <= P.LmaxT) v = P.srcT[L][mi];
+    if (c == 1 && P.has_E && L <= P.LmaxP) v = P.srcE[L][mi];
+    if (c == 2 && P.has_B && L <= P.LmaxP) v = P.srcB[L][mi];
+    tp[L] = v;
+  }
+  const long tstep = (long)pr.nc * nt;
+  const double *scol = P.src + pr.src_off + t;
+  const double *dcol = P.dd + pr.src_off + t;
+  const bool cubic = P.cubic_k3 != 0;
+  double s0[NL], s1[NL], d0[NL], d1[NL];
+  auto load = [&](int kc, double *sv, double *dv) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const long o = (long)max(tp[u][c], 0) * tstep;
-        s0[u][c] = col[o];
-        s1[u][c] = col[o + ntau];
-        if (P.cubic_k3) {
-          d0[u][c] = dcol[o];
-          d1[u][c] = dcol[o + ntau];
-        }
-      }
+    for (int L = 0; L < NL; ++L) {
+      const long o = (long)max(tp[L], 0) * tstep + (long)kc * nt;
+      sv[L] = scol[o];
+      if (cubic) dv[L] = dcol[o];
     }
+  };
+  const double dt = P.dtau[t];
+  const double t0mt = P.t0mt[t];
+  const bool do_ixa = mi == 0 && c == 0;
+  int cur = -2;
+  for (int f = pr.f_off; f < pr.f_off + pr.nf; ++f) {
+    const int br = P.fbr[f];
+    if (br != cur) {
+      if (br == cur + 1) {
 #pragma unroll
-    for (int u = 0; u < QB; ++u) {
-      if (q0 + u >= nq) break;
-      double v[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        double w;
-        if (P.cubic_k3)
-          w = aw * s0[u][c] + bw * s1[u][c] + (ca * d0[u][c] + cb * d1[u][c]) * h * h / 6.0;
-        else
-          w = aw * s0[u][c] + bw * s1[u][c];
-        if (WITH_DT) w = w * dt;
-        v[c] = tp[u][c] >= 0 ? w : 0.0;
+        for (int L = 0; L < NL; ++L) {
+          s0[L] = s1[L];
+          d0[L] = d1[L];
+        }
+        load(br + 1, s1, d1);
+      } else {
+        load(br, s0, d0);
+        load(br + 1, s1, d1);
       }
-      SweepS<R> o;
-      o.T = (R)v[0];
-      o.E = (R)v[1];
-      o.B = (R)v[2];
-      o.pad = 0;
-      out[q0 + u] = o;
+      cur = br;
+    }
+    const double *kc = P.kc + pr.kc_off;
+    const double ks = P.kf[P.fsrc[f]];
+    const double h = kc[br + 1] - kc[br];
+    const double bw = (ks - kc[br]) / h;
+    const double aw = 1 - bw;
+    const double ca = aw * aw * aw - aw, cb = bw * bw * bw - bw;
+    R *o = (R *)P.sf + ((((long)f * nt + t) * P.m_size + mi) * 3 + c) * NL;
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+      double v = cubic ? aw * s0[L] + bw * s1[L] + (ca * d0[L] + cb * d1[L]) * h * h / 6.0
+                       : aw * s0[L] + bw * s1[L];
+      if (WITH_DT) v = v * dt;
+      o[L] = (R)(tp[L] >= 0 ? v : 0.0);
+    }
+    if (do_ixa) {
+      const double k = P.kf[f];
+      const double x = k * t0mt;
+      const int ix = (int)(x / P.xx_step);
+      SweepIXA<R> w;
+      w.ix = ix;
+      w.a = (R)((P.xx[ix + 1] - x) / P.xx_step);
+      ((SweepIXA<R> *)P.ixa)[(long)f * nt + t] = w;
     }
   }
 }
@@ -242,7 +248,7 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
   const unsigned xstep = (unsigned)NLP * esz;
   const unsigned rowb = (unsigned)P.X * xstep;
   const SweepIXA<R> *__restrict__ ixa = (const SweepIXA<R> *)P.ixa;
-  const SweepS<R> *__restrict__ sf = (const SweepS<R> *)P.sf;
+  const R *__restrict__ sf = (const R *)P.sf;
   const bool doT = P.has_T, doP = P.has_E || P.has_B, mix = P.jEB >= 0;
   const unsigned bT = doT ? (unsigned)(P.jTT * P.L_size) * rowb : 0u;
   const unsigned bE = doP ? (unsigned)(P.jEE * P.L_size) * rowb : 0u;
@@ -288,23 +294,24 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
             }
           }
         }
-        const SweepS<R> *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * NL;
+        // S x dtau of this element and step: [field T, E, B][L]
+        const R *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * 3 * NL;
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
           if (m > L) continue;
-          const SweepS<R> s = sp[L];
+          const R sT = sp[L], sE = sp[NL + L], sB = sp[2 * NL + L];
           if (doT && L <= P.LmaxT) {
             R Jv = tv[L].j1 + a[j] * (tv[L].j0 - tv[L].j1);
             Jv = ix[j] >= xmT[L] ? Jv : (R)0;
-            pT[j] += Jv * s.T;
+            pT[j] += Jv * sT;
           }
           if (doP && L <= P.LmaxP) {
             R Je = ev[L].j1 + a[j] * (ev[L].j0 - ev[L].j1);
             R Jm = qv[L].j1 + a[j] * (qv[L].j0 - qv[L].j1);
             Je = ix[j] >= xmE[L] ? Je : (R)0;
             Jm = ix[j] >= xmM[L] ? Jm : (R)0;
-            pE[j] += Je * s.E + Jm * (emix * s.B);
-            pB[j] += Je * s.B - Jm * s.E;
+            pE[j] += Je * sE + Jm * (emix * sB);
+            pB[j] += Je * sB - Jm * sE;
           }
         }
       }
@@ -362,12 +369,18 @@ los_sweep(los_params P)
 template <class R, bool WITH_DT>
 static hipError_t launch_sweep_prep(const los_params *P, hipStream_t s)
 {
-  const long n = (long)P->F * P->ntau;
-  if (n == 0) return hipSuccess;
+  const long n = (long)P->n_pairs * P->m_size * 3 * P->ntau;
+  if (n == 0 || P->F == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
   const long nb = (n + 255) / 256, per = (nb + 7) / 8;
-  hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT>), dim3((unsigned)(8 * per)), dim3(256), 0, s, *P,
-                     NL, (int)per);
+  const dim3 grid((unsigned)(8 * per));
+  switch (NL) {
+#define LOS_SP(k) \
+  case k: hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT, k>), grid, dim3(256), 0, s, *P, (int)per); break;
+    LOS_SP(1) LOS_SP(2) LOS_SP(3) LOS_SP(4) LOS_SP(5) LOS_SP(6) LOS_SP(7) LOS_SP(8)
+#undef LOS_SP
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -476,7 +489,7 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
   const unsigned xstep = (unsigned)NLP * 16u;
   const unsigned rowb = (unsigned)P.X * xstep;
   const SweepIXA<double> *__restrict__ ixa = (const SweepIXA<double> *)P.ixa;
-  const SweepS<double> *__restrict__ sf = (const SweepS<double> *)P.sf;
+  const double *__restrict__ sf = (const double *)P.sf;
   const bool doT = P.has_T && P.jTT >= 0, doP = (P.has_E || P.has_B) && P.jEE >= 0;
   const int jEB = P.jEB >= 0 ? P.jEB : P.jEE;
   const unsigned bT = doT ? (unsigned)(P.jTT * P.L_size) * rowb : 0u;
@@ -514,29 +527,30 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
           }
         }
       }
-      const SweepS<double> *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * NL;
+      // interpolated sources of this element and step: [field T, E, B][L]
+      const double *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * 3 * NL;
       const double aJ = a[j];
       if (doT) {
         double integrand = 0;
 #pragma unroll
         for (int L = 0; L < NL; ++L)
-          if (ix[j] >= xmT[L]) integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sp[L].T;
+          if (ix[j] >= xmT[L]) integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sp[L];
         accT[j] += integrand * dt;
       }
       if (doP) {
         double iEd = 0, iEm = 0, iBd = 0, iBm = 0;
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
-          const SweepS<double> s = sp[L];
+          const double sE = sp[NL + L], sB = sp[2 * NL + L];
           if (ix[j] >= xmE[L]) {
             const double Jv = aJ * ev[L].j0 + (1 - aJ) * ev[L].j1;
-            if (P.has_E) iEd += Jv * s.E;
-            if (P.has_B) iBd += Jv * s.B;
+            if (P.has_E) iEd += Jv * sE;
+            if (P.has_B) iBd += Jv * sB;
           }
           if (ix[j] >= xmM[L]) {
             const double Jv = aJ * qv[L].j0 + (1 - aJ) * qv[L].j1;
-            if (m != 0) iEm += Jv * s.B;
-            iBm += Jv * s.E;
+            if (m != 0) iEm += Jv * sB;
+            iBm += Jv * sE;
           }
         }
         accEd[j] += iEd * dt;

@@ -54,7 +54,7 @@ class CConfig(C.Structure):
         ("seed", C.c_ulonglong),
         ("tau_peak", C.c_double), ("tau_sigma", C.c_double), ("ramp_amp", C.c_double),
         ("ramp_center", C.c_double), ("ramp_width", C.c_double),
-        ("verbose", C.c_int),
+        ("verbose", C.c_int), ("j_build", C.c_int),
     ]
 
 
@@ -78,6 +78,8 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_scenario_scatter": (i, [vp, C.POINTER(d), C.c_long, ch]),
         "los_scenario_k3_sizes": (i, [vp, C.POINTER(i), C.c_long, ch]),
         "los_bessel2_build": (i, [vp] * 5 + [ch]),
+        "los_bessel2_build_gpu": (i, [vp] * 5 + [i, ch]),
+        "los_scenario_rebuild_J": (i, [vp, i, i, C.POINTER(d), ch]),
         "los_sph_bessel_array": (None, [i, d, C.POINTER(d)]),
         "los_wigner3j_j1": (i, [i, i, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(d), i]),
         "transfer2_init": (i, [vp] * 10),
@@ -189,6 +191,7 @@ class Config:
     ramp_center: float = 7000.0
     ramp_width: float = 2000.0
     verbose: int = 0
+    j_build: int = 0  # J tables: 0 host builder, 1 GPU builder (device 0)
 
     def to_c(self) -> CConfig:
         c = CConfig()
@@ -340,6 +343,14 @@ class Scenario:
         xmin, xs = C.c_int(), C.c_int()
         p = self.lib.los_scenario_J_row(self.ptr, jt, L, il, im, C.byref(xmin), C.byref(xs))
         return xmin.value, np.ctypeslib.as_array(p, shape=(xs.value,))
+
+    def rebuild_J(self, gpu: bool, device: int = 0) -> float:
+        """Build the J tables again on the host or on the GPU; returns the seconds taken."""
+        sec = C.c_double()
+        err = _err()
+        check(self.lib.los_scenario_rebuild_J(self.ptr, int(bool(gpu)), device, C.byref(sec), err),
+              err, "los_scenario_rebuild_J")
+        return sec.value
 
     def free_transfers(self, fn=None) -> None:
         if self._inited:
