@@ -85,7 +85,8 @@ def kernel_name(form: str, shape, nl: int) -> str:
                 f"los_klane_exact<{nl}, {ch}>" if kind == 2 else f"los_integrate_exact<{nl}>")
     pol = POLICY[form]
     if kind == 1:
-        return f"los_sweep<{pol}, {nk}, {nl}, {flag}>"
+        pairs = "true" if (form == "fp64fast" or ch == 1) else "false"
+        return f"los_sweep<{pol}, {nk}, {nl}, {flag}, {pairs}>"
     if kind == 2:
         return f"los_klane<{pol}, {nl}, {ch}>"
     return f"los_integrate_stream{'_q' if pers else ''}<{pol}, {nk}, {ch}>"
@@ -388,7 +389,8 @@ def main(argv=None):
             res["valu_tflops"] = flop_live / (los_ms * 1e-3) / 1e12
         kind, nk, ch, pers = eng.get_shape(FORMS[form])
         res["kernel"] = kernel_name(form, (kind, nk, ch, pers), max(cfg.l_max_los_t, cfg.l_max_los_p) + 1)
-        res["tile_shape"] = ({"kernel": "sweep", "nk": nk, "persistent": pers} if kind == 1 else
+        res["tile_shape"] = ({"kernel": "sweep", "nk": nk, "persistent": pers,
+                              "j_pairs": bool(form != "fp32c" or ch == 1)} if kind == 1 else
                              {"kernel": "k-lane", "l_per_lane": ch} if kind == 2 else
                              {"kernel": "workgroup per k3"} if form == "fp64" else
                              {"kernel": "stream", "nk": nk, "ch": ch, "persistent": pers})

@@ -689,9 +689,10 @@ int enqueue_prep(los_engine *e, char *err)
 }
 
 // Whether the sweep kernels can address the batch's J table (32-bit offsets).
-bool sweep_fits(const los_engine *e, int form)
+bool sweep_fits(const los_engine *e, int form, const los_shape &sh)
 {
-  const double bytes = (double)e->nJp * (form == LOS_FORM_FP32C ? sizeof(float) : 2 * sizeof(double));
+  const bool fp32 = form == LOS_FORM_FP32C, pairs = !fp32 || sh.ch == 1;
+  const double bytes = (double)e->nJp * (fp32 ? sizeof(float) : sizeof(double)) * (pairs ? 2 : 1);
   return bytes <= (double)LOS_BUF_MAX;
 }
 
@@ -713,7 +714,7 @@ static int form_shapes(const los_engine *e, int form, los_shape *out, int cap)
   }
   int k = 0;
   for (int i = 0; i < n && k < cap; ++i)
-    if (all[i].kind != 1 || sweep_fits(e, form)) out[k++] = all[i];
+    if (all[i].kind != 1 || sweep_fits(e, form, all[i])) out[k++] = all[i];
   return k;
 }
 
@@ -757,9 +758,10 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   }
   split_xcd(tiles, cost, P->xcd_tile);
   if (sh->kind == 1) {
-    P->Jp = fp32 ? (const void *)e->dJp32 : e->dJp2[0];
-    P->Jp_bytes = (unsigned)(e->nJp * (fp32 ? sizeof(float) : 2 * sizeof(double)));
-    HIPCHK(los_launch_sweep(fp32, P, sh->nk, sh->persistent, e->n_cu, s), err);
+    const bool pairs = !fp32 || sh->ch == 1;
+    P->Jp = !fp32 ? e->dJp2[0] : pairs ? e->dJp2[1] : (const void *)e->dJp32;
+    P->Jp_bytes = (unsigned)(e->nJp * (fp32 ? sizeof(float) : sizeof(double)) * (pairs ? 2 : 1));
+    HIPCHK(los_launch_sweep(fp32, P, sh->nk, sh->persistent, pairs, e->n_cu, s), err);
   } else {
     HIPCHK(los_launch_stream(fp32, P, sh, e->n_cu, s), err);
   }
@@ -822,7 +824,7 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
 {
   if (sh == nullptr || sh->kind != 1) return _SUCCESS_;
   const int fp32 = e->form == LOS_FORM_FP32C;
-  if (!sweep_fits(e, e->form)) {
+  if (!sweep_fits(e, e->form, *sh)) {
     snprintf(err, _ERRORMSGSIZE_,
              "sweep shape: the J table of this batch (%zu entries) exceeds the 32-bit offsets "
              "of the sweep kernels; use another kernel organisation",
@@ -835,11 +837,20 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
     HIPCHK(hipStreamSynchronize(e->stream), err);
   }
   if (!fp32 && e->dJp2[0] == nullptr) {
-    // FP64: (J[x], J[x+1]) pairs, one load per row
+    // FP64: (J[x], J[x+1]) pairs, zero below each row's index_xmin_J, one load per row
     const int NLP = e->m_size * e->n_lp;
     HIPCHK(hipMalloc(&e->dJp2[0], std::max<size_t>(1, e->nJp) * 2 * sizeof(double)), err);
-    HIPCHK(los_launch_sweep_pairs(0, e->dJp, e->dJp2[0], (long)e->J_size * e->L_size, e->X, NLP,
-                                  e->stream),
+    HIPCHK(los_launch_sweep_pairs(0, e->dJp, e->dJp2[0], e->dxminp, (long)e->J_size * e->L_size,
+                                  e->X, NLP, e->stream),
+           err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+  }
+  if (fp32 && sh->ch == 1 && e->dJp2[1] == nullptr) {
+    // FP32 pairs of the rounded table, same zeroing
+    const int NLP = e->m_size * e->n_lp;
+    HIPCHK(hipMalloc(&e->dJp2[1], std::max<size_t>(1, e->nJp) * 2 * sizeof(float)), err);
+    HIPCHK(los_launch_sweep_pairs(1, e->dJp32, e->dJp2[1], e->dxminp, (long)e->J_size * e->L_size,
+                                  e->X, NLP, e->stream),
            err);
     HIPCHK(hipStreamSynchronize(e->stream), err);
   }
@@ -986,7 +997,8 @@ extern "C" int los_engine_set_shape(los_engine *e, int form, int kind, int nk, i
   for (int i = 0; i < nc; ++i)
     if (cand[i].kind == kind &&
         (kind == 2   ? cand[i].ch == ch
-         : kind == 1 ? (cand[i].nk == nk && cand[i].persistent == (persistent != 0))
+         : kind == 1 ? (cand[i].nk == nk && cand[i].persistent == (persistent != 0) &&
+                        cand[i].ch == ch)
                      : (form == LOS_FORM_FP64 ||
                         (cand[i].nk == nk && cand[i].ch == ch &&
                          cand[i].persistent == (persistent != 0))))) {

@@ -22,13 +22,14 @@ extern "C" hipError_t los_launch_stream(int fp32, const los_params *P, const los
 extern "C" hipError_t los_stream_set_attrs(int tp2_size);
 extern "C" int los_shapes(int fp32, int tp2_size, los_shape *out, int cap);
 extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s);
-extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int n_cu,
-                                       hipStream_t s);
+extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int pairs,
+                                       int n_cu, hipStream_t s);
 extern "C" hipError_t los_launch_sweep_prep_exact(const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_cu,
                                              hipStream_t s);
-extern "C" hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, long rows, int X,
-                                             int NLP, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst,
+                                             const int *xminp, long rows, int X, int NLP,
+                                             hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep(int fp32, const los_params *P, hipStream_t s);
 extern "C" hipError_t los_launch_klane(int fp32, const los_params *P, int nlb, hipStream_t s);
 extern "C" hipError_t los_launch_klane_prep_exact(const los_params *P, hipStream_t s);
@@ -159,7 +160,7 @@ std::vector<los_tile> make_tiles(const std::vector<int> &job_end, int nk);
 int enqueue_prep(los_engine *e, char *err);
 int ensure_j32(los_engine *e, char *err);
 int current_shape(los_engine *e, const los_shape **sh, char *err);
-bool sweep_fits(const los_engine *e, int form);
+bool sweep_fits(const los_engine *e, int form, const los_shape &sh);
 int ensure_sweep(los_engine *e, const los_shape *sh, char *err);
 int make_sweep_tables(los_engine *e, char *err);
 int make_klane_tables(los_engine *e, char *err);

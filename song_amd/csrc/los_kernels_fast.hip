@@ -83,33 +83,33 @@ hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s)
 }
 
 // Pair table of the sweep kernels from the sweep J table.
-hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, long rows, int X, int NLP,
-                                  hipStream_t s)
+hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, const int *xminp,
+                                  long rows, int X, int NLP, hipStream_t s)
 {
   if (rows * X * NLP == 0) return hipSuccess;
   if (fp32)
     hipLaunchKernelGGL(los_sweep_pairs<float>, dim3(4096), dim3(256), 0, s, (const float *)src,
-                       (SweepPair<float> *)dst, rows, X, NLP);
+                       (SweepPair<float> *)dst, xminp, rows, X, NLP);
   else
     hipLaunchKernelGGL(los_sweep_pairs<double>, dim3(4096), dim3(256), 0, s, (const double *)src,
-                       (SweepPair<double> *)dst, rows, X, NLP);
+                       (SweepPair<double> *)dst, xminp, rows, X, NLP);
   return hipGetLastError();
 }
 
-hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int n_cu,
+hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int pairs, int n_cu,
                             hipStream_t s)
 {
-  const bool p = pers != 0;
+  const bool p = pers != 0, q = pairs != 0;
   switch (nk) {
     case 1:
-      return fp32 ? launch_sweep<StreamFP32C, 1>(P, p, n_cu, s)
-                  : launch_sweep<StreamFP64, 1>(P, p, n_cu, s);
+      return fp32 ? launch_sweep<StreamFP32C, 1>(P, p, q, n_cu, s)
+                  : launch_sweep<StreamFP64, 1>(P, p, q, n_cu, s);
     case 2:
-      return fp32 ? launch_sweep<StreamFP32C, 2>(P, p, n_cu, s)
-                  : launch_sweep<StreamFP64, 2>(P, p, n_cu, s);
+      return fp32 ? launch_sweep<StreamFP32C, 2>(P, p, q, n_cu, s)
+                  : launch_sweep<StreamFP64, 2>(P, p, q, n_cu, s);
     case 4:
-      return fp32 ? launch_sweep<StreamFP32C, 4>(P, p, n_cu, s)
-                  : launch_sweep<StreamFP64, 4>(P, p, n_cu, s);
+      return fp32 ? launch_sweep<StreamFP32C, 4>(P, p, q, n_cu, s)
+                  : launch_sweep<StreamFP64, 4>(P, p, q, n_cu, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -155,8 +155,10 @@ int los_shapes(int fp32, int tp2_size, los_shape *out, int cap)
     for (int p = 0; p < 2 && n < cap; ++p) out[n++] = los_shape{0, NK_, CH_, p};
   LOS_SHAPES(LOS_LIST)
 #undef LOS_LIST
+  // sweep: ch = 1 reads the FP32 pair table (FP64 always reads its pair table)
   for (int nk = 1; nk <= 4; nk *= 2)
-    for (int p = 0; p < 2 && n < cap; ++p) out[n++] = los_shape{1, nk, 0, p};
+    for (int c = 0; c <= fp32; ++c)
+      for (int p = 0; p < 2 && n < cap; ++p) out[n++] = los_shape{1, nk, c, p};
   for (int b = 1; b <= 4 && n < cap; b *= 2) out[n++] = los_shape{2, 64, b, 0};
   return n;
 }

@@ -334,7 +334,8 @@ extern "C" int los_server_create(int device, int form, double hbm_budget_bytes, 
   for (int sp = 0; sp < e->F; ++sp) S->fcost[e->hforder[sp]] = pc[sp];
   S->resident = (size_t)(16.0 * e->src_total) + e->nJ * 8 + e->nJp * 8 + e->nJk * 8 +
                 (e->dJp32 ? e->nJp * 4 : 0) + (e->dJ32 ? e->nJ * 4 : 0) +
-                (e->dJk32 ? e->nJk * 4 : 0) + e->sweep_bytes + e->klane_bytes;
+                (e->dJk32 ? e->nJk * 4 : 0) + (e->dJp2[0] ? e->nJp * 16 : 0) +
+                (e->dJp2[1] ? e->nJp * 8 : 0) + e->sweep_bytes + e->klane_bytes;
 
   S->slots.resize(n_slots);
   for (slot_rec &sl : S->slots) {

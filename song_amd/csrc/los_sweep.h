@@ -69,6 +69,13 @@ __device__ __forceinline__ SweepPair<double> sweep_row<double>(__amdgpu_buffer_r
   return __builtin_bit_cast(SweepPair<double>,
                             __builtin_amdgcn_raw_buffer_load_b128(J, lane, off, 0));
 }
+// FP32 pair table: one 8-byte load.
+__device__ __forceinline__ SweepPair<float> sweep_pair_f32(__amdgpu_buffer_rsrc_t J, unsigned lane,
+                                                          unsigned off)
+{
+  return __builtin_bit_cast(SweepPair<float>,
+                            __builtin_amdgcn_raw_buffer_load_b64(J, lane, off, 0));
+}
 template <>
 __device__ __forceinline__ SweepPair<float> sweep_row<float>(__amdgpu_buffer_rsrc_t J,
                                                              unsigned lane, unsigned off,
@@ -80,19 +87,38 @@ __device__ __forceinline__ SweepPair<float> sweep_row<float>(__amdgpu_buffer_rsr
   return p;
 }
 
-// Pair table [row][x][lp] from the sweep table [row][x][lp] (x + 1 = X reads 0).
+template <class R, bool PAIRS>
+__device__ __forceinline__ SweepPair<R> sweep_load(__amdgpu_buffer_rsrc_t J, unsigned lane,
+                                                   unsigned off, unsigned xstep)
+{
+  if constexpr (sizeof(R) == 4 && PAIRS)
+    return sweep_pair_f32(J, lane, off);
+  else
+    return sweep_row<R>(J, lane, off, xstep);
+}
+
+// Pair table [row][x][lp] from the sweep table [row][x][lp]: (J[x], J[x+1])
+// where x >= index_xmin_J of (row, lp), (0, 0) below it (x + 1 = X reads 0).
+// A time step whose index_x lies below the row's index_xmin_J, which the
+// reference skips (source/transfer2.c:2232), then adds exactly zero: the
+// kernels reading this table need no per-row test.
 template <class R>
 __global__ void __launch_bounds__(256)
-los_sweep_pairs(const R *src, SweepPair<R> *dst, long rows, int X, int NLP)
+los_sweep_pairs(const R *src, SweepPair<R> *dst, const int *xminp, long rows, int X, int NLP)
 {
   const long n = rows * X * NLP;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const long lp = i % NLP;
     const long rx = i / NLP;
     const int x = (int)(rx % X);
+    const long r = rx / X;
     SweepPair<R> p;
-    p.j0 = src[i];
-    p.j1 = x + 1 < X ? src[i + NLP] : (R)0;
+    if (x >= xminp[r * NLP + lp]) {
+      p.j0 = src[i];
+      p.j1 = x + 1 < X ? src[i + NLP] : (R)0;
+    } else {
+      p.j0 = p.j1 = (R)0;
+    }
     dst[i] = p;
   }
 }
@@ -103,26 +129,31 @@ los_sweep_pairs(const R *src, SweepPair<R> *dst, long rows, int X, int NLP)
 // interpolated in k3 (source/transfer2.c:2700-2706), times the trapezoid
 // weight (source/transfer2.c:2317) in the fast forms:
 //   ixa[f][t] = {index_x, a_J},   sf[f][t][mi][field][L] (field T, E, B).
-// One thread per (pair, mi, field, t), t fastest: it walks the pair's fine k3
-// in order and keeps the two coarse columns of the current bracket (sources
-// and their spline second derivatives, for every L) in registers; moving to
-// the next bracket loads only the new right-hand column, so every coarse
-// source column is read once. Workgroup b runs on XCD b % 8 and takes logical
-// block (b % 8) * per + b / 8 (each XCD a contiguous range of pairs).
+// A workgroup takes one pair and LOS_SP_TB consecutive time steps; its
+// threads are (mi, field, t), t fastest. They walk the pair's fine k3 in
+// order, each keeping the two coarse columns of the current bracket (sources
+// and spline second derivatives, every L) in registers, so that moving to the
+// next bracket loads only the new right-hand column and every coarse column is
+// read once. Per fine k3 the values are staged in LDS in the table's order and
+// the workgroup writes its LOS_SP_TB records as one contiguous block.
 // ---------------------------------------------------------------------------
+#define LOS_SP_TB 32
+#define LOS_SP_MAXT (LOS_SP_TB * 3 * LOS_MAX_MI)
+
 template <class R, bool WITH_DT, int NL>
-__global__ void __launch_bounds__(256)
-los_sweep_prep(los_params P, int per)
+__global__ void __launch_bounds__(LOS_SP_MAXT)
+los_sweep_prep(los_params P, int ntb)
 {
-  const long blk = (long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  const long id = blk * blockDim.x + threadIdx.x;
+  HIP_DYNAMIC_SHARED(double, stage_raw)
+  R *stage = (R *)stage_raw;  // [tt][mi][field][L]
+  const int nmc = P.m_size * 3;
+  const int p = blockIdx.x / ntb;
+  const int t0 = (blockIdx.x - p * ntb) * LOS_SP_TB;
+  const int tt = threadIdx.x % LOS_SP_TB, mc = threadIdx.x / LOS_SP_TB;
   const long nt = P.ntau;
-  const long per_pair = (long)P.m_size * 3 * nt;
-  if (id >= (long)P.n_pairs * per_pair) return;
-  const int p = (int)(id / per_pair);
-  const long r = id - (long)p * per_pair;
-  const int mc = (int)(r / nt);
-  const int t = (int)(r - (long)mc * nt);
+  const int nlive = min(LOS_SP_TB, (int)(nt - t0));
+  const bool live = tt < nlive;
+  const int t = t0 + (live ? tt : 0);
   const int mi = mc / 3, c = mc - mi * 3;
   const los_pair pr = P.pairs[p];
 
@@ -151,7 +182,8 @@ los_sweep_prep(los_params P, int per)
   };
   const double dt = P.dtau[t];
   const double t0mt = P.t0mt[t];
-  const bool do_ixa = mi == 0 && c == 0;
+  const int nval = nlive * nmc * NL;
+  R *st = stage + (tt * nmc + mc) * NL;
   int cur = -2;
   for (int f = pr.f_off; f < pr.f_off + pr.nf; ++f) {
     const int br = P.fbr[f];
@@ -175,15 +207,14 @@ los_sweep_prep(los_params P, int per)
     const double bw = (ks - kc[br]) / h;
     const double aw = 1 - bw;
     const double ca = aw * aw * aw - aw, cb = bw * bw * bw - bw;
-    R *o = (R *)P.sf + ((((long)f * nt + t) * P.m_size + mi) * 3 + c) * NL;
 #pragma unroll
     for (int L = 0; L < NL; ++L) {
       double v = cubic ? aw * s0[L] + bw * s1[L] + (ca * d0[L] + cb * d1[L]) * h * h / 6.0
                        : aw * s0[L] + bw * s1[L];
       if (WITH_DT) v = v * dt;
-      o[L] = (R)(tp[L] >= 0 ? v : 0.0);
+      st[L] = (R)(tp[L] >= 0 ? v : 0.0);
     }
-    if (do_ixa) {
+    if (mc == 0 && live) {
       const double k = P.kf[f];
       const double x = k * t0mt;
       const int ix = (int)(x / P.xx_step);
@@ -192,6 +223,10 @@ los_sweep_prep(los_params P, int per)
       w.a = (R)((P.xx[ix + 1] - x) / P.xx_step);
       ((SweepIXA<R> *)P.ixa)[(long)f * nt + t] = w;
     }
+    __syncthreads();
+    R *dst = (R *)P.sf + ((long)f * nt + t0) * nmc * NL;
+    for (int i = threadIdx.x; i < nval; i += blockDim.x) dst[i] = stage[i];
+    __syncthreads();
   }
 }
 
@@ -199,7 +234,10 @@ los_sweep_prep(los_params P, int per)
 // Sweep kernel. Block = one wave; blockIdx.y = wave class (mi, l chunk);
 // blockIdx.x = XCD-aware tile index (tiles of XCD b % 8, costliest first).
 // ---------------------------------------------------------------------------
-template <class Pol, int NK, int NL>
+// PAIRS: the J table is a pair table zeroed below each row's index_xmin_J
+// (always for FP64; an option for FP32), so no per-row test is needed;
+// otherwise (FP32 plain table) the interpolated J is selected to zero there.
+template <class Pol, int NK, int NL, bool PAIRS>
 __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, int w)
 {
   typedef typename Pol::real R;
@@ -233,7 +271,7 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
 #pragma unroll
   for (int L = 0; L < NL; ++L) {
     xmT[L] = xmE[L] = xmM[L] = INT_MAX;
-    if (!active || m > L || L >= P.L_size) continue;
+    if (PAIRS || !active || m > L || L >= P.L_size) continue;
     if (P.has_T && L <= P.LmaxT && P.jTT >= 0)
       xmT[L] = P.xminp[((long)P.jTT * P.L_size + L) * NLP + lp];
     if ((P.has_E || P.has_B) && L <= P.LmaxP) {
@@ -241,9 +279,9 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
       if (P.jEB >= 0) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
     }
   }
-  // FP64: pair table, FP32: plain table (both [row][x][lp])
+  // pair table (FP64, or FP32 with PAIRS) or FP32 plain table, all [row][x][lp]
   const __amdgpu_buffer_rsrc_t Jr = sweep_rsrc(P.Jp, P.Jp_bytes);
-  constexpr unsigned esz = sizeof(R) == 8 ? 16u : 4u;  // bytes of one x slot of one lane
+  constexpr unsigned esz = PAIRS ? 2u * sizeof(R) : (unsigned)sizeof(R);  // bytes per x slot
   const unsigned lane = (unsigned)lp * esz;
   const unsigned xstep = (unsigned)NLP * esz;
   const unsigned rowb = (unsigned)P.X * xstep;
@@ -287,10 +325,10 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
 #pragma unroll
           for (int L = 0; L < NL; ++L) {
             if (m > L) continue;
-            if (doT && L <= P.LmaxT) tv[L] = sweep_row<R>(Jr, lane, bT + L * rowb + ox, xstep);
+            if (doT && L <= P.LmaxT) tv[L] = sweep_load<R, PAIRS>(Jr, lane, bT + L * rowb + ox, xstep);
             if (doP && L <= P.LmaxP) {
-              ev[L] = sweep_row<R>(Jr, lane, bE + L * rowb + ox, xstep);
-              if (mix) qv[L] = sweep_row<R>(Jr, lane, bQ + L * rowb + ox, xstep);
+              ev[L] = sweep_load<R, PAIRS>(Jr, lane, bE + L * rowb + ox, xstep);
+              if (mix) qv[L] = sweep_load<R, PAIRS>(Jr, lane, bQ + L * rowb + ox, xstep);
             }
           }
         }
@@ -302,14 +340,16 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
           const R sT = sp[L], sE = sp[NL + L], sB = sp[2 * NL + L];
           if (doT && L <= P.LmaxT) {
             R Jv = tv[L].j1 + a[j] * (tv[L].j0 - tv[L].j1);
-            Jv = ix[j] >= xmT[L] ? Jv : (R)0;
+            if (!PAIRS) Jv = ix[j] >= xmT[L] ? Jv : (R)0;
             pT[j] += Jv * sT;
           }
           if (doP && L <= P.LmaxP) {
             R Je = ev[L].j1 + a[j] * (ev[L].j0 - ev[L].j1);
             R Jm = qv[L].j1 + a[j] * (qv[L].j0 - qv[L].j1);
-            Je = ix[j] >= xmE[L] ? Je : (R)0;
-            Jm = ix[j] >= xmM[L] ? Jm : (R)0;
+            if (!PAIRS) {
+              Je = ix[j] >= xmE[L] ? Je : (R)0;
+              Jm = ix[j] >= xmM[L] ? Jm : (R)0;
+            }
             pE[j] += Je * sE + Jm * (emix * sB);
             pB[j] += Je * sB - Jm * sE;
           }
@@ -357,11 +397,11 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
   }
 
 // Sweep kernel, one wave per workgroup (see sweep_item).
-template <class Pol, int NK, int NL, bool PERS>
+template <class Pol, int NK, int NL, bool PERS, bool PAIRS>
 __global__ void __launch_bounds__(64)
 los_sweep(los_params P)
 {
-#define LOS_CALL_FAST(...) sweep_item<Pol, NK, NL>(__VA_ARGS__)
+#define LOS_CALL_FAST(...) sweep_item<Pol, NK, NL, PAIRS>(__VA_ARGS__)
   LOS_SWEEP_ITEMS(LOS_CALL_FAST)
 #undef LOS_CALL_FAST
 }
@@ -369,14 +409,14 @@ los_sweep(los_params P)
 template <class R, bool WITH_DT>
 static hipError_t launch_sweep_prep(const los_params *P, hipStream_t s)
 {
-  const long n = (long)P->n_pairs * P->m_size * 3 * P->ntau;
-  if (n == 0 || P->F == 0) return hipSuccess;
+  if (P->n_pairs == 0 || P->F == 0 || P->ntau == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  const long nb = (n + 255) / 256, per = (nb + 7) / 8;
-  const dim3 grid((unsigned)(8 * per));
+  const int ntb = (P->ntau + LOS_SP_TB - 1) / LOS_SP_TB;
+  const dim3 grid((unsigned)((long)P->n_pairs * ntb)), block(LOS_SP_TB * 3 * P->m_size);
+  const size_t lds = (size_t)LOS_SP_TB * 3 * P->m_size * NL * sizeof(R);
   switch (NL) {
 #define LOS_SP(k) \
-  case k: hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT, k>), grid, dim3(256), 0, s, *P, (int)per); break;
+  case k: hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT, k>), grid, block, lds, s, *P, ntb); break;
     LOS_SP(1) LOS_SP(2) LOS_SP(3) LOS_SP(4) LOS_SP(5) LOS_SP(6) LOS_SP(7) LOS_SP(8)
 #undef LOS_SP
     default: return hipErrorInvalidValue;
@@ -412,24 +452,30 @@ static hipError_t sweep_grid(const los_params *P, bool pers, const void *k, int 
   return hipSuccess;
 }
 
-// P->tiles / P->xcd_tile must describe tiles of at most NK positions.
+// P->tiles / P->xcd_tile must describe tiles of at most NK positions. FP64
+// always reads the pair table; FP32 reads the pair table when `pairs`.
 template <class Pol, int NK>
-static hipError_t launch_sweep(const los_params *P, bool pers, int n_cu, hipStream_t s)
+static hipError_t launch_sweep(const los_params *P, bool pers, bool pairs, int n_cu, hipStream_t s)
 {
   if (P->n_pos == 0 || P->n_wave == 0) return hipSuccess;
+  constexpr bool f64 = sizeof(typename Pol::real) == 8;
+  if (f64) pairs = true;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  switch (NL * 2 + (pers ? 1 : 0)) {
-#define LOS_SW1(n, b)                                                                 \
-  case n * 2 + b: {                                                                   \
-    dim3 grid;                                                                        \
-    const hipError_t e =                                                              \
-        sweep_grid(P, b, (const void *)los_sweep<Pol, NK, n, b>, n_cu, &grid);        \
-    if (e != hipSuccess) return e;                                                    \
-    if (grid.x == 0) return hipSuccess;                                               \
-    hipLaunchKernelGGL((los_sweep<Pol, NK, n, b>), grid, dim3(64), 0, s, *P);         \
-    break;                                                                            \
+  switch (NL * 4 + (pers ? 2 : 0) + (pairs ? 1 : 0)) {
+#define LOS_SW1(n, b, q)                                                                \
+  case n * 4 + (b ? 2 : 0) + (q ? 1 : 0): {                                             \
+    if (f64 && !q) return hipErrorInvalidValue;                                         \
+    constexpr bool qq = f64 ? true : q;                                                 \
+    dim3 grid;                                                                          \
+    const hipError_t e =                                                                \
+        sweep_grid(P, b, (const void *)los_sweep<Pol, NK, n, b, qq>, n_cu, &grid);      \
+    if (e != hipSuccess) return e;                                                      \
+    if (grid.x == 0) return hipSuccess;                                                 \
+    hipLaunchKernelGGL((los_sweep<Pol, NK, n, b, qq>), grid, dim3(64), 0, s, *P);       \
+    break;                                                                              \
   }
-#define LOS_SW(n) LOS_SW1(n, false) LOS_SW1(n, true)
+#define LOS_SW(n) LOS_SW1(n, false, false) LOS_SW1(n, false, true) LOS_SW1(n, true, false) \
+                  LOS_SW1(n, true, true)
     LOS_SW(1) LOS_SW(2) LOS_SW(3) LOS_SW(4) LOS_SW(5) LOS_SW(6) LOS_SW(7) LOS_SW(8)
 #undef LOS_SW
 #undef LOS_SW1
@@ -471,19 +517,13 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
     f[j] = P.forder[tl.pos0 + min(j, tl.npos - 1)];
     tend = max(tend, P.tau_end_fw[(long)f[j] * P.n_wave + w]);
   }
-  // rows the reference reads for this (l,m) (INT_MAX: skipped at every tau)
-  int xmT[NL], xmE[NL], xmM[NL];
-#pragma unroll
-  for (int L = 0; L < NL; ++L) {
-    xmT[L] = xmE[L] = xmM[L] = INT_MAX;
-    if (!active || m > L || L >= P.L_size) continue;
-    if (P.has_T && L <= P.LmaxT && P.jTT >= 0)
-      xmT[L] = P.xminp[((long)P.jTT * P.L_size + L) * NLP + lp];
-    if ((P.has_E || P.has_B) && L <= P.LmaxP) {
-      if (P.jEE >= 0) xmE[L] = P.xminp[((long)P.jEE * P.L_size + L) * NLP + lp];
-      if (P.jEB >= 0 && P.has_B) xmM[L] = P.xminp[((long)P.jEB * P.L_size + L) * NLP + lp];
-    }
-  }
+  // The pair table is zero below each row's index_xmin_J (los_sweep_pairs), so
+  // a time step the reference skips for a row (source/transfer2.c:2232) adds
+  // (a_J 0 + (1 - a_J) 0) S = +-0 here, which leaves the sum's bits unchanged
+  // (a sum that starts at +0 never becomes -0); rows the reference never
+  // reads (m > L, L beyond the field's L_max, the B->E mixing without B
+  // sources) are skipped by wave-uniform tests.
+  const bool mixB = P.jEB >= 0 && P.has_B;
   const __amdgpu_buffer_rsrc_t Jr = sweep_rsrc(P.Jp, P.Jp_bytes);
   const unsigned lane = (unsigned)lp * 16u;
   const unsigned xstep = (unsigned)NLP * 16u;
@@ -534,7 +574,7 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
         double integrand = 0;
 #pragma unroll
         for (int L = 0; L < NL; ++L)
-          if (ix[j] >= xmT[L]) integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sp[L];
+          if (m <= L && L <= P.LmaxT) integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sp[L];
         accT[j] += integrand * dt;
       }
       if (doP) {
@@ -542,12 +582,13 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
           const double sE = sp[NL + L], sB = sp[2 * NL + L];
-          if (ix[j] >= xmE[L]) {
+          if (m > L || L > P.LmaxP) continue;
+          {
             const double Jv = aJ * ev[L].j0 + (1 - aJ) * ev[L].j1;
             if (P.has_E) iEd += Jv * sE;
             if (P.has_B) iBd += Jv * sB;
           }
-          if (ix[j] >= xmM[L]) {
+          if (mixB) {
             const double Jv = aJ * qv[L].j0 + (1 - aJ) * qv[L].j1;
             if (m != 0) iEm += Jv * sB;
             iBm += Jv * sE;

@@ -14,8 +14,12 @@ from song_amd import lib as L
 from tests.conftest import field_errors, run_oracle
 from tests.test_gpu_parity import CASES
 
-SHAPES = ([(0, nk, ch, p) for nk in (2, 4, 8) for ch in (16, 32) for p in (0, 1)] +
-          [(1, nk, 0, p) for nk in (1, 2, 4) for p in (0, 1)] + [(2, 64, b, 0) for b in (1, 2, 4)])
+_STREAM = [(0, nk, ch, p) for nk in (2, 4, 8) for ch in (16, 32) for p in (0, 1)]
+_KLANE = [(2, 64, b, 0) for b in (1, 2, 4)]
+# sweep: ch = 1 is the FP32 pair table (the FP64 forms always read their pair table)
+SHAPES = {L.FORM_FP64_FAST: _STREAM + [(1, nk, 0, p) for nk in (1, 2, 4) for p in (0, 1)] + _KLANE,
+          L.FORM_FP32C: _STREAM + [(1, nk, c, p) for nk in (1, 2, 4) for c in (0, 1)
+                                   for p in (0, 1)] + _KLANE}
 # the plain form's kernels: per-k3 workgroups, sweep (dispatch and persistent), k-lane
 FP64_SHAPES = ([(0, 1, 0, 0)] + [(1, nk, 0, p) for nk in (1, 2, 4) for p in (0, 1)] +
                [(2, 64, b, 0) for b in (1, 2, 4)])
@@ -55,7 +59,7 @@ def test_every_tile_shape(oracle):
     tt2 = _tt2(s, ref)
     for form, tol in ((L.FORM_FP64_FAST, 1e-12), (L.FORM_FP32C, 1e-5)):
         eng.set_form(form)
-        for sh in SHAPES:
+        for sh in SHAPES[form]:
             eng.set_shape(form, *sh)
             print(f"form {form} shape {sh}", flush=True)
             err = field_errors(_run(eng, s, form), ref, tt2)
@@ -112,7 +116,8 @@ def test_uneven_k3_counts(oracle):
         print(f"form {form} tuned shape {eng.get_shape(form)}", flush=True)
         err = field_errors(_run(eng, s, form), ref, tt2)
         assert err.max() < tol, f"form {form}: {err.max():.3e}"
-        for sh in ((0, 4, 32, 1), (1, 2, 0, 0), (1, 4, 0, 1), (2, 64, 4, 0)):
+        for sh in ((0, 4, 32, 1), (1, 2, 0, 0), (1, 4, 0, 1), (2, 64, 4, 0)) + \
+                (((1, 2, 1, 0),) if form == L.FORM_FP32C else ()):
             eng.set_shape(form, *sh)
             err = field_errors(_run(eng, s, form), ref, tt2)
             assert err.max() < tol, f"form {form} shape {sh}: {err.max():.3e}"
@@ -125,7 +130,7 @@ def test_uneven_k3_counts(oracle):
                                         (L.FORM_FP64, (1, 2)), (L.FORM_FP64, (1, 2, 0, 1)),
                                         (L.FORM_FP32C, (0, 4, 32, 1)),
                                         (L.FORM_FP64_FAST, (0, 8, 16, 0)),
-                                        (L.FORM_FP32C, (1, 2, 0, 0)),
+                                        (L.FORM_FP32C, (1, 2, 0, 0)), (L.FORM_FP32C, (1, 2, 1, 0)),
                                         (L.FORM_FP32C, (2, 64, 4))])
 def test_graph_replay_matches_eager(form, shape):
     cfg = L.Config(**CASES["TEB_m012"])
