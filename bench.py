@@ -368,13 +368,11 @@ def main(argv=None):
         prep_ms = min(k[1] for k in kern)
         if graph:
             eng.capture()
-            eng.run()
+        eng.run()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             eng.run()
         wall = time.perf_counter() - t0
-        if not graph:
-            los_ms, prep_ms = eng.last_kernel_ms()
         eng.store(scen)
         tt_out = scen.gather()
         tt2 = tt_out.size // int(scen.k3_sizes()[:, 0].sum())

@@ -73,9 +73,19 @@ int los_engine_load_pairs(los_engine *eng, struct precision *ppr, struct precisi
    Runs the captured graph when los_engine_capture was called for this form. */
 int los_engine_run(los_engine *eng, float *ms, char *errmsg);
 
+/* Pipelined steps (off by default): with on = n >= 2, the next load splits the
+   batch into n chunks of whole pairs, and a step of a sweep tile shape runs each
+   chunk's source re-layout, k3 spline and per-(k3, tau) tables on a second stream
+   while the previous chunk's LOS kernel runs; los_engine_run then reports the step
+   time only. on = 0 or 1: one chunk, one pass of each kernel, timed separately.
+   On MI355X the two parts share the texture/L2 path of the gathers and the
+   chunked order costs the LOS kernel locality, so the default stays off.
+   No counterpart in SONG. */
+int los_engine_set_pipeline(los_engine *e, int on, char *errmsg);
+
 /* GPU time of the last eager run's LOS kernel and of everything before it (source
    re-layout, k3 spline, per-(k3,tau) tables of the kernel shape), in ms; -1 after
-   a graph run (only the total is known). */
+   a graph or pipelined run (only the total is known). */
 int los_engine_last_kernel_ms(los_engine *eng, float *ms_los, float *ms_spline);
 
 /* Kernel organisation and tile shape of a form (enum los_form).

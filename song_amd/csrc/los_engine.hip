@@ -126,7 +126,9 @@ extern "C" int los_engine_create(int device, los_engine **out, char *err)
   los_engine *e = new los_engine();
   e->device = device;
   HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), err);
+  HIPCHK(hipStreamCreateWithFlags(&e->stream2, hipStreamNonBlocking), err);
   for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&e->ev[i]), err);
+  HIPCHK(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming), err);
   e->staging_bytes = 64ul << 20;
   HIPCHK(hipHostMalloc((void **)&e->staging, e->staging_bytes, hipHostMallocDefault), err);
   HIPCHK(hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, device), err);
@@ -146,6 +148,12 @@ extern "C" void los_engine_destroy(los_engine *e)
   if (e->staging) (void)hipHostFree(e->staging);
   for (int i = 0; i < 4; ++i)
     if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+  if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+  for (hipEvent_t v : e->ev_chunk) (void)hipEventDestroy(v);
+  if (e->stream2) {
+    (void)hipStreamSynchronize(e->stream2);
+    (void)hipStreamDestroy(e->stream2);
+  }
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }
@@ -480,10 +488,40 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
     }
   }
   e->F = (int)kf.size();
-  // processing order: fine k3 sorted by value over all pairs (ties by index)
-  std::vector<int> forder(e->F);
+  // chunks of whole pairs with about equal fine k3 counts (pipelined steps)
+  {
+    const int np = (int)e->pairs.size();
+    const int C = std::max(1, std::min(e->chunks_forced, np));
+    e->chunk_pair.assign(1, 0);
+    e->chunk_f.assign(1, 0);
+    for (int c = 1; c < C; ++c) {
+      const long goal = (long)e->F * c / C;
+      int q = e->chunk_pair.back();
+      while (q < np && e->pairs[q].f_off < goal) ++q;
+      if (q > e->chunk_pair.back() && q < np) {
+        e->chunk_pair.push_back(q);
+        e->chunk_f.push_back(e->pairs[q].f_off);
+      }
+    }
+    e->chunk_pair.push_back(np);
+    e->chunk_f.push_back(e->F);
+    e->n_chunk = (int)e->chunk_pair.size() - 1;
+    while ((int)e->ev_chunk.size() < e->n_chunk) {
+      hipEvent_t v;
+      HIPCHK(hipEventCreateWithFlags(&v, hipEventDisableTiming), err);
+      e->ev_chunk.push_back(v);
+    }
+  }
+  // processing order: chunk by chunk, fine k3 sorted by value inside a chunk
+  // (ties by index)
+  std::vector<int> forder(e->F), chunk_of(e->F, 0);
+  for (int c = 0; c < e->n_chunk; ++c)
+    for (int f = e->chunk_f[c]; f < e->chunk_f[c + 1]; ++f) chunk_of[f] = c;
   for (int i = 0; i < e->F; ++i) forder[i] = i;
-  std::stable_sort(forder.begin(), forder.end(), [&](int a, int b) { return kf[a] < kf[b]; });
+  std::stable_sort(forder.begin(), forder.end(), [&](int a, int b) {
+    return chunk_of[a] != chunk_of[b] ? chunk_of[a] < chunk_of[b] : kf[a] < kf[b];
+  });
+  e->chunk_pos = e->chunk_f;  // chunk c holds positions chunk_f[c] .. chunk_f[c+1]-1
   e->src_total = src_off;
   e->ht0mt = t0mt;
   e->hforder = forder;
@@ -546,7 +584,14 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   jobs[0].ttE = ptr2->index_tt2_E;
   jobs[0].ttB = ptr2->index_tt2_B;
   for (int i = 0; i < 5; ++i) {
-    e->htiles[i] = make_tiles(std::vector<int>(1, e->F), i < 4 ? 1 << i : 64);
+    // tiles never straddle two chunks; chunk_tile[i] holds each chunk's first tile
+    e->htiles[i] = make_tiles(std::vector<int>(e->chunk_pos.begin() + 1, e->chunk_pos.end()),
+                              i < 4 ? 1 << i : 64);
+    e->chunk_tile[i].assign(e->n_chunk + 1, (int)e->htiles[i].size());
+    for (int c = 0, t = 0; c < e->n_chunk; ++c) {
+      while (t < (int)e->htiles[i].size() && e->htiles[i][t].pos0 < e->chunk_pos[c]) ++t;
+      e->chunk_tile[i][c] = t;
+    }
     if (upload_vec(e, e->dtiles[i], e->htiles[i], err)) return _FAILURE_;
   }
   if (upload_vec(e, e->dposjob, posjob, err) || upload_vec(e, e->docol, forder, err) ||
@@ -770,18 +815,21 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
 
 // Per-(k3, tau) tables of a sweep or k-lane shape for the loaded batch
 // (enqueued on the engine stream).
-static int enqueue_tables(los_engine *e, const los_shape *sh, char *err)
+static int enqueue_tables(los_engine *e, const los_shape *sh, char *err, int f_begin = 0,
+                          int f_end = -1, hipStream_t s = nullptr)
 {
   const int fp32 = e->form == LOS_FORM_FP32C;
+  if (f_end < 0) f_end = e->F;
+  if (!s) s = e->stream;
   if (sh->kind == 1) {
     if (e->sweep_form != e->form) {
       snprintf(err, _ERRORMSGSIZE_, "enqueue_tables: sweep tables not allocated for this form");
       return _FAILURE_;
     }
     if (e->form == LOS_FORM_FP64)
-      HIPCHK(los_launch_sweep_prep_exact(&e->P, e->stream), err);
+      HIPCHK(los_launch_sweep_prep_exact(&e->P, f_begin, f_end, s), err);
     else
-      HIPCHK(los_launch_sweep_prep(fp32, &e->P, e->stream), err);
+      HIPCHK(los_launch_sweep_prep(fp32, &e->P, f_begin, f_end, s), err);
   } else if (sh->kind == 2) {
     if (e->klane_form != e->form) {
       snprintf(err, _ERRORMSGSIZE_, "enqueue_tables: k-lane tables not allocated for this form");
@@ -808,6 +856,42 @@ static int enqueue_los(los_engine *e, const los_shape *sh, char *err, hipEvent_t
   const std::vector<double> &cost =
       sh->kind == 1 ? e->pos_cost_sweep : e->pos_cost;
   return enqueue_los_params(e, sh, &P, e->stream, e->htiles[slot], cost, err);
+}
+
+// Whether a step of shape sh runs pipelined (chunk c+1's preparation on
+// stream2 while chunk c's LOS kernel runs): sweep shapes of batches with more
+// than one chunk.
+static bool pipelined(const los_engine *e, const los_shape *sh)
+{
+  return sh->kind == 1 && e->n_chunk > 1;
+}
+
+// One pipelined step: per chunk, the source re-layout and spline of its pairs
+// and its sweep tables on stream2, then (after an event) its LOS kernel on the
+// engine stream. stream2 first waits for everything already on the engine
+// stream, so a step never overwrites tables that the previous step still reads.
+static int enqueue_pipelined(los_engine *e, const los_shape *sh, char *err)
+{
+  const int tp2 = e->P.tp2_size;
+  const int slot = nk_slot(sh->nk);
+  HIPCHK(hipEventRecord(e->ev_fork, e->stream), err);
+  HIPCHK(hipStreamWaitEvent(e->stream2, e->ev_fork, 0), err);
+  for (int c = 0; c < e->n_chunk; ++c) {
+    const int p0 = e->chunk_pair[c], p1 = e->chunk_pair[c + 1];
+    HIPCHK(los_launch_relayout_spline(e->dpairs + p0, p1 - p0, e->nc_max, e->dkc, e->dspl, e->draw,
+                                      e->dsrc, e->ddd, e->ntau, tp2, e->P.cubic_k3, e->stream2),
+           err);
+    if (enqueue_tables(e, sh, err, e->chunk_f[c], e->chunk_f[c + 1], e->stream2))
+      return _FAILURE_;
+    HIPCHK(hipEventRecord(e->ev_chunk[c], e->stream2), err);
+    HIPCHK(hipStreamWaitEvent(e->stream, e->ev_chunk[c], 0), err);
+    los_params P = e->P;
+    const int t0 = e->chunk_tile[slot][c], t1 = e->chunk_tile[slot][c + 1];
+    P.tiles = e->dtiles[slot] + t0;
+    const std::vector<los_tile> tiles(e->htiles[slot].begin() + t0, e->htiles[slot].begin() + t1);
+    if (enqueue_los_params(e, sh, &P, e->stream, tiles, e->pos_cost_sweep, err)) return _FAILURE_;
+  }
+  return _SUCCESS_;
 }
 
 // Per-(k3, tau) tables of the current shape for every loaded fine k3 (waits).
@@ -1043,8 +1127,13 @@ extern "C" int los_engine_capture(los_engine *e, char *err)
   if (ensure_j32(e, err) || current_shape(e, &sh, err)) return _FAILURE_;
   drop_graph(e);
   HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), err);
-  int rc = enqueue_prep(e, err);
-  if (rc == _SUCCESS_) rc = enqueue_los(e, sh, err);
+  int rc;
+  if (pipelined(e, sh) && e->pipeline) {
+    rc = enqueue_pipelined(e, sh, err);
+  } else {
+    rc = enqueue_prep(e, err);
+    if (rc == _SUCCESS_) rc = enqueue_los(e, sh, err);
+  }
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(e->stream, &g);
   if (rc != _SUCCESS_) {
@@ -1080,6 +1169,19 @@ extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
     if (ms) *ms = t;
     return _SUCCESS_;
   }
+  if (pipelined(e, sh) && e->pipeline) {
+    // pipelined step: only the total time is known
+    HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
+    if (enqueue_pipelined(e, sh, err)) return _FAILURE_;
+    HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
+    HIPCHK(hipEventSynchronize(e->ev[2]), err);
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e->ev[0], e->ev[2]), err);
+    e->ms_spline = -1;
+    e->ms_los = -1;
+    if (ms) *ms = t;
+    return _SUCCESS_;
+  }
   // ev0 .. ev1: source re-layout, k3 spline and the shape's per-(k3, tau)
   // tables; ev1 .. ev2: the LOS kernel alone
   HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
@@ -1090,6 +1192,18 @@ extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
   HIPCHK(hipEventElapsedTime(&e->ms_spline, e->ev[0], e->ev[1]), err);
   HIPCHK(hipEventElapsedTime(&e->ms_los, e->ev[1], e->ev[2]), err);
   if (ms) *ms = e->ms_spline + e->ms_los;
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_set_pipeline(los_engine *e, int on, char *err)
+{
+  if (on < 0) {
+    snprintf(err, _ERRORMSGSIZE_, "los_engine_set_pipeline: %d chunks", on);
+    return _FAILURE_;
+  }
+  if ((on >= 2) != e->pipeline) drop_graph(e);
+  e->pipeline = on >= 2;
+  e->chunks_forced = on >= 2 ? on : 0;
   return _SUCCESS_;
 }
 

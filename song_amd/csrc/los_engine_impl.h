@@ -21,10 +21,12 @@ extern "C" hipError_t los_launch_stream(int fp32, const los_params *P, const los
                                         int n_cu, hipStream_t s);
 extern "C" hipError_t los_stream_set_attrs(int tp2_size);
 extern "C" int los_shapes(int fp32, int tp2_size, los_shape *out, int cap);
-extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_prep(int fp32, const los_params *P, int f_begin,
+                                            int f_end, hipStream_t s);
 extern "C" hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int pairs,
                                        int n_cu, hipStream_t s);
-extern "C" hipError_t los_launch_sweep_prep_exact(const los_params *P, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_prep_exact(const los_params *P, int f_begin, int f_end,
+                                                  hipStream_t s);
 extern "C" hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_cu,
                                              hipStream_t s);
 extern "C" hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst,
@@ -42,6 +44,16 @@ struct los_engine {
   int form = LOS_FORM_FP64;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float ms_spline = 0, ms_los = 0;
+
+  // pipelined steps: the batch in chunks of whole pairs; chunk c's re-layout,
+  // spline and sweep tables run on stream2 while chunk c-1's LOS kernel runs
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_chunk;
+  int chunks_forced = 0;              // chunk count set by the caller (0: one chunk)
+  int n_chunk = 1;                    // chunk count of the loaded batch
+  std::vector<int> chunk_pair, chunk_f, chunk_pos;  // [n_chunk + 1] boundaries
+  std::vector<int> chunk_tile[5];     // [n_chunk + 1] tile boundaries per tile-size slot
 
   // projection tables (resident across loads)
   const void *J_owner = nullptr;
@@ -93,6 +105,7 @@ struct los_engine {
   int tt2_size = 0;
   los_params P;
   bool loaded = false;
+  bool pipeline = false;  // pipelined steps (chunks_forced >= 2 and a sweep shape)
 
   double *staging = nullptr;  // pinned
   size_t staging_bytes = 0;

@@ -398,9 +398,9 @@ hipError_t los_launch_relayout_spline(const los_pair *pairs, int npairs, int nc_
 }
 
 // sweep organisation of the plain FP64 form (same per-element operations)
-hipError_t los_launch_sweep_prep_exact(const los_params *P, hipStream_t s)
+hipError_t los_launch_sweep_prep_exact(const los_params *P, int f_begin, int f_end, hipStream_t s)
 {
-  return launch_sweep_prep<double, false>(P, s);
+  return launch_sweep_prep<double, false>(P, f_begin, f_end, s);
 }
 
 hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_cu, hipStream_t s)

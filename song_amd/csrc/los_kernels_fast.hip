@@ -77,9 +77,11 @@ hipError_t los_stream_set_attrs(int tp2_size)
 
 // Sweep organisation: per-(k3, tau) tables, then the sweep kernel with tiles of
 // nk positions (1, 2 or 4).
-hipError_t los_launch_sweep_prep(int fp32, const los_params *P, hipStream_t s)
+hipError_t los_launch_sweep_prep(int fp32, const los_params *P, int f_begin, int f_end,
+                                 hipStream_t s)
 {
-  return fp32 ? launch_sweep_prep<float, true>(P, s) : launch_sweep_prep<double, true>(P, s);
+  return fp32 ? launch_sweep_prep<float, true>(P, f_begin, f_end, s)
+              : launch_sweep_prep<double, true>(P, f_begin, f_end, s);
 }
 
 // Pair table of the sweep kernels from the sweep J table.

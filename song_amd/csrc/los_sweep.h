@@ -129,33 +129,34 @@ los_sweep_pairs(const R *src, SweepPair<R> *dst, const int *xminp, long rows, in
 // interpolated in k3 (source/transfer2.c:2700-2706), times the trapezoid
 // weight (source/transfer2.c:2317) in the fast forms:
 //   ixa[f][t] = {index_x, a_J},   sf[f][t][mi][field][L] (field T, E, B).
-// A workgroup takes one pair and LOS_SP_TB consecutive time steps; its
-// threads are (mi, field, t), t fastest. They walk the pair's fine k3 in
-// order, each keeping the two coarse columns of the current bracket (sources
-// and spline second derivatives, every L) in registers, so that moving to the
-// next bracket loads only the new right-hand column and every coarse column is
-// read once. Per fine k3 the values are staged in LDS in the table's order and
-// the workgroup writes its LOS_SP_TB records as one contiguous block.
+// A workgroup takes LOS_SP_FC consecutive fine k3 (fine indices are grouped by
+// pair) and LOS_SP_TB consecutive time steps; its threads are (mi, field, t),
+// t fastest. They walk the fine k3 in order, each keeping the two coarse
+// columns of the current bracket (sources and spline second derivatives,
+// every L) in registers, so that moving to the next bracket loads only the new
+// right-hand column. Per fine k3 the values are staged in LDS in the table's
+// order and the workgroup writes its LOS_SP_TB records as one contiguous block.
 // ---------------------------------------------------------------------------
 #define LOS_SP_TB 32
+#define LOS_SP_FC 16
 #define LOS_SP_MAXT (LOS_SP_TB * 3 * LOS_MAX_MI)
 
 template <class R, bool WITH_DT, int NL>
 __global__ void __launch_bounds__(LOS_SP_MAXT)
-los_sweep_prep(los_params P, int ntb)
+los_sweep_prep(los_params P, int ntb, int f_begin, int f_end)
 {
   HIP_DYNAMIC_SHARED(double, stage_raw)
   R *stage = (R *)stage_raw;  // [tt][mi][field][L]
   const int nmc = P.m_size * 3;
-  const int p = blockIdx.x / ntb;
-  const int t0 = (blockIdx.x - p * ntb) * LOS_SP_TB;
+  const int fc = blockIdx.x / ntb;
+  const int t0 = (blockIdx.x - fc * ntb) * LOS_SP_TB;
+  const int f0 = f_begin + fc * LOS_SP_FC, f1 = min(f_end, f0 + LOS_SP_FC);
   const int tt = threadIdx.x % LOS_SP_TB, mc = threadIdx.x / LOS_SP_TB;
   const long nt = P.ntau;
   const int nlive = min(LOS_SP_TB, (int)(nt - t0));
   const bool live = tt < nlive;
   const int t = t0 + (live ? tt : 0);
   const int mi = mc / 3, c = mc - mi * 3;
-  const los_pair pr = P.pairs[p];
 
   // source type of each L for this (m, field); absent ones read type 0 and give 0
   int tp[NL];
@@ -167,26 +168,44 @@ los_sweep_prep(los_params P, int ntb)
     if (c == 2 && P.has_B && L <= P.LmaxP) v = P.srcB[L][mi];
     tp[L] = v;
   }
-  const long tstep = (long)pr.nc * nt;
-  const double *scol = P.src + pr.src_off + t;
-  const double *dcol = P.dd + pr.src_off + t;
   const bool cubic = P.cubic_k3 != 0;
-  double s0[NL], s1[NL], d0[NL], d1[NL];
-  auto load = [&](int kc, double *sv, double *dv) {
-#pragma unroll
-    for (int L = 0; L < NL; ++L) {
-      const long o = (long)max(tp[L], 0) * tstep + (long)kc * nt;
-      sv[L] = scol[o];
-      if (cubic) dv[L] = dcol[o];
-    }
-  };
   const double dt = P.dtau[t];
   const double t0mt = P.t0mt[t];
   const int nval = nlive * nmc * NL;
   R *st = stage + (tt * nmc + mc) * NL;
-  int cur = -2;
-  for (int f = pr.f_off; f < pr.f_off + pr.nf; ++f) {
-    const int br = P.fbr[f];
+
+  double s0[NL], s1[NL], d0[NL], d1[NL];
+  int cur = -2, curp = -1;
+  long tstep = 0;
+  const double *scol = nullptr, *dcol = nullptr, *kc = nullptr;
+  auto load = [&](int k, double *sv, double *dv) {
+#pragma unroll
+    for (int L = 0; L < NL; ++L) {
+      const long o = (long)max(tp[L], 0) * tstep + (long)k * nt;
+      sv[L] = scol[o];
+      if (cubic) dv[L] = dcol[o];
+    }
+  };
+  // the next fine k3's pair, bracket and interpolation point, read one step ahead
+  int np = P.fpair[f0], nbr = P.fbr[f0];
+  double nks = P.kf[P.fsrc[f0]];
+  for (int f = f0; f < f1; ++f) {
+    const int pi = np, br = nbr;
+    const double ks = nks;
+    if (f + 1 < f1) {
+      np = P.fpair[f + 1];
+      nbr = P.fbr[f + 1];
+      nks = P.kf[P.fsrc[f + 1]];
+    }
+    if (pi != curp) {
+      const los_pair pr = P.pairs[pi];
+      tstep = (long)pr.nc * nt;
+      scol = P.src + pr.src_off + t;
+      dcol = P.dd + pr.src_off + t;
+      kc = P.kc + pr.kc_off;
+      curp = pi;
+      cur = -2;
+    }
     if (br != cur) {
       if (br == cur + 1) {
 #pragma unroll
@@ -201,8 +220,6 @@ los_sweep_prep(los_params P, int ntb)
       }
       cur = br;
     }
-    const double *kc = P.kc + pr.kc_off;
-    const double ks = P.kf[P.fsrc[f]];
     const double h = kc[br + 1] - kc[br];
     const double bw = (ks - kc[br]) / h;
     const double aw = 1 - bw;
@@ -406,17 +423,22 @@ los_sweep(los_params P)
 #undef LOS_CALL_FAST
 }
 
+// Tables of the fine k3 f_begin .. f_end - 1.
 template <class R, bool WITH_DT>
-static hipError_t launch_sweep_prep(const los_params *P, hipStream_t s)
+static hipError_t launch_sweep_prep(const los_params *P, int f_begin, int f_end, hipStream_t s)
 {
-  if (P->n_pairs == 0 || P->F == 0 || P->ntau == 0) return hipSuccess;
+  if (P->n_pairs == 0 || f_end <= f_begin || P->ntau == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
   const int ntb = (P->ntau + LOS_SP_TB - 1) / LOS_SP_TB;
-  const dim3 grid((unsigned)((long)P->n_pairs * ntb)), block(LOS_SP_TB * 3 * P->m_size);
+  const long nfc = ((long)(f_end - f_begin) + LOS_SP_FC - 1) / LOS_SP_FC;
+  const dim3 grid((unsigned)(nfc * ntb)), block(LOS_SP_TB * 3 * P->m_size);
   const size_t lds = (size_t)LOS_SP_TB * 3 * P->m_size * NL * sizeof(R);
   switch (NL) {
 #define LOS_SP(k) \
-  case k: hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT, k>), grid, block, lds, s, *P, ntb); break;
+  case k:                                                                                  \
+    hipLaunchKernelGGL((los_sweep_prep<R, WITH_DT, k>), grid, block, lds, s, *P, ntb, f_begin,  \
+                       f_end);                                                                \
+    break;
     LOS_SP(1) LOS_SP(2) LOS_SP(3) LOS_SP(4) LOS_SP(5) LOS_SP(6) LOS_SP(7) LOS_SP(8)
 #undef LOS_SP
     default: return hipErrorInvalidValue;

@@ -96,6 +96,7 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_load": (i, [vp] * 7 + [i, i, ch]),
         "los_engine_load_pairs": (i, [vp] * 7 + [C.POINTER(i), C.POINTER(i), i, ch]),
         "los_engine_autotune": (i, [vp, i, ch]),
+        "los_engine_set_pipeline": (i, [vp, i, ch]),
         "los_engine_set_shape": (i, [vp, i, i, i, i, i, ch]),
         "los_engine_get_shape": (None, [vp, i] + [C.POINTER(i)] * 4),
         "los_engine_capture": (i, [vp, ch]),
@@ -409,6 +410,11 @@ class Engine:
         a, b = C.c_float(), C.c_float()
         self.lib.los_engine_last_kernel_ms(self.ptr, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_pipeline(self, on: int) -> None:
+        """Pipelined steps (chunked preparation overlapping the LOS kernel): n >= 2 chunks
+        from the next load; 0 or 1 off (the default)."""
+        self._call("los_engine_set_pipeline", int(on))
 
     def autotune(self, reps: int = 2) -> None:
         self._call("los_engine_autotune", reps)

@@ -193,3 +193,36 @@ def test_flop_count_matches_brute_force():
     eng.close()
     s.close()
     assert live == tot and dense == dtot
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form,shape", [(L.FORM_FP64, (1, 2, 0, 0)), (L.FORM_FP64, (1, 4, 0, 0)),
+                                        (L.FORM_FP64_FAST, (1, 4, 0, 0)),
+                                        (L.FORM_FP32C, (1, 2, 1, 0)), (L.FORM_FP32C, (1, 4, 0, 0))])
+def test_pipelined_steps(oracle, form, shape):
+    """Chunked steps (preparation of chunk c+1 on a second stream while chunk c's LOS
+    kernel runs), eager and captured, equal the oracle as the single-pass steps do."""
+    cfg = L.Config(**CASES["TEB_m012"])
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    s = L.Scenario(cfg)
+    s.prepare()
+    eng = L.Engine(0)
+    eng.set_pipeline(3)
+    eng.load(s)
+    eng.set_form(form)
+    eng.set_shape(form, *shape)
+    tt2 = _tt2(s, ref)
+    for graph in (False, True):
+        if graph:
+            eng.capture()
+        for rep in range(2):
+            got = _run(eng, s, form)
+            print(f"form {form} shape {shape} graph {graph} rep {rep}", flush=True)
+            if form == L.FORM_FP64:
+                assert np.array_equal(got, ref)
+            else:
+                tol = 1e-12 if form == L.FORM_FP64_FAST else 1e-5
+                assert field_errors(got, ref, tt2).max() < tol
+    eng.close()
+    s.close()
+
