@@ -403,6 +403,15 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
     }
   free_batch(e);
   e->shape_set[0] = e->shape_set[1] = e->shape_set[2] = false;  // shapes are chosen per batch
+  if (npairs == 0) {
+    // an empty batch: nothing to compute; run, capture, autotune and store do nothing
+    e->pairs.clear();
+    e->F = 0;
+    e->n_chunk = 1;
+    e->tt2_size = ptr2->tt2_size;
+    e->loaded = true;
+    return _SUCCESS_;
+  }
   const int ntau = ppt2->tau_size;
   const int tp2 = ppt2->tp2_size;
 
@@ -1020,6 +1029,7 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
     snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: nothing loaded");
     return _FAILURE_;
   }
+  if (e->F == 0) return _SUCCESS_;
   HIPCHK(hipSetDevice(e->device), err);
   const int form = e->form;
   const tune_key key = key_of(e);
@@ -1122,6 +1132,7 @@ extern "C" int los_engine_capture(los_engine *e, char *err)
     snprintf(err, _ERRORMSGSIZE_, "los_engine_capture: nothing loaded");
     return _FAILURE_;
   }
+  if (e->F == 0) return _SUCCESS_;  // empty batch: runs do nothing
   HIPCHK(hipSetDevice(e->device), err);
   const los_shape *sh;
   if (ensure_j32(e, err) || current_shape(e, &sh, err)) return _FAILURE_;
@@ -1152,6 +1163,11 @@ extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
   if (!e->loaded) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_run: nothing loaded");
     return _FAILURE_;
+  }
+  if (e->F == 0) {
+    e->ms_spline = e->ms_los = 0;
+    if (ms) *ms = 0;
+    return _SUCCESS_;
   }
   HIPCHK(hipSetDevice(e->device), err);
   const los_shape *sh;
@@ -1220,6 +1236,7 @@ extern "C" int los_engine_store(los_engine *e, struct transfers2 *ptr2, char *er
     snprintf(err, _ERRORMSGSIZE_, "los_engine_store: nothing loaded");
     return _FAILURE_;
   }
+  if (e->F == 0) return _SUCCESS_;
   HIPCHK(hipSetDevice(e->device), err);
   const size_t n = (size_t)e->tt2_size * e->F;
   std::vector<double> h(n);
