@@ -97,6 +97,16 @@ __device__ __forceinline__ SweepPair<R> sweep_load(__amdgpu_buffer_rsrc_t J, uns
     return sweep_row<R>(J, lane, off, xstep);
 }
 
+// Value of lane i of v for the whole wave (v_readlane; every lane must call it).
+__device__ __forceinline__ double sweep_lane(double v, int i)
+{
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, i);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), i);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ int sweep_lane(int v, int i) { return __builtin_amdgcn_readlane(v, i); }
+
 // Pair table [row][x][lp] from the sweep table [row][x][lp]: (J[x], J[x+1])
 // where x >= index_xmin_J of (row, lp), (0, 0) below it (x + 1 = X reads 0).
 // A time step whose index_x lies below the row's index_xmin_J, which the
@@ -565,16 +575,32 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
   SweepPair<double> tv[NL], ev[NL], qv[NL];
 #pragma unroll
   for (int L = 0; L < NL; ++L) tv[L] = ev[L] = qv[L] = SweepPair<double>{0., 0.};
+  // The per-step values that all lanes share come through the vector path and
+  // are broadcast with readlane where they are used: lane j holds element j's
+  // (index_x, a_J), read one step ahead, and lane i of src[j] holds value i of
+  // element j's interpolated sources [field T, E, B][L] of the current step,
+  // whose loads go out together with the step's J rows (scalar loads would
+  // wait one after the other).
+  const int li = threadIdx.x;
+  int fl = f[0];  // this lane's element (lanes below NK)
+#pragma unroll
+  for (int j = 1; j < NK; ++j) fl = li == j ? f[j] : fl;
+  SweepIXA<double> nxt{0, 0, 0.};
+  if (li < NK && tend > 0) nxt = ixa[(long)fl * P.ntau];
   for (int t = 0; t < tend; ++t) {
     const double dt = P.dtau[t];
     int ix[NK];
     double a[NK];
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
-      const SweepIXA<double> v = ixa[(long)f[j] * P.ntau + t];
-      ix[j] = v.ix;
-      a[j] = v.a;
+      ix[j] = sweep_lane(nxt.ix, j);
+      a[j] = sweep_lane(nxt.a, j);
     }
+    if (li < NK && t + 1 < tend) nxt = ixa[(long)fl * P.ntau + t + 1];
+    double src[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j)
+      src[j] = li < 3 * NL ? sf[(((long)f[j] * P.ntau + t) * P.m_size + mi) * 3 * NL + li] : 0.;
 #pragma unroll
     for (int j = 0; j < NK; ++j) {
       if (j == 0 || ix[j] != ix[j - 1]) {
@@ -589,22 +615,21 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
           }
         }
       }
-      // interpolated sources of this element and step: [field T, E, B][L]
-      const double *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * 3 * NL;
       const double aJ = a[j];
       if (doT) {
         double integrand = 0;
 #pragma unroll
         for (int L = 0; L < NL; ++L)
-          if (m <= L && L <= P.LmaxT) integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sp[L];
+          if (m <= L && L <= P.LmaxT)
+            integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sweep_lane(src[j], L);
         accT[j] += integrand * dt;
       }
       if (doP) {
         double iEd = 0, iEm = 0, iBd = 0, iBm = 0;
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
-          const double sE = sp[NL + L], sB = sp[2 * NL + L];
           if (m > L || L > P.LmaxP) continue;
+          const double sE = sweep_lane(src[j], NL + L), sB = sweep_lane(src[j], 2 * NL + L);
           {
             const double Jv = aJ * ev[L].j0 + (1 - aJ) * ev[L].j1;
             if (P.has_E) iEd += Jv * sE;
