@@ -106,6 +106,10 @@ __device__ __forceinline__ double sweep_lane(double v, int i)
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 __device__ __forceinline__ int sweep_lane(int v, int i) { return __builtin_amdgcn_readlane(v, i); }
+__device__ __forceinline__ float sweep_lane(float v, int i)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), i));
+}
 
 // Pair table [row][x][lp] from the sweep table [row][x][lp]: (J[x], J[x+1])
 // where x >= index_xmin_J of (row, lp), (0, 0) below it (x + 1 = X reads 0).
@@ -327,6 +331,16 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
   SweepPair<R> tv[NL], ev[NL], qv[NL];
 #pragma unroll
   for (int L = 0; L < NL; ++L) tv[L] = ev[L] = qv[L] = SweepPair<R>{(R)0, (R)0};
+  // Per-step values shared by all lanes come through the vector path and are
+  // broadcast with readlane where used: lane j holds element j's (index_x, a_J),
+  // read one step ahead; lane i of src[j] holds value i of element j's
+  // [field T, E, B][L] sources of the current step, loaded with the J rows.
+  const int li = threadIdx.x;
+  int fl = f[0];  // this lane's element (lanes below NK)
+#pragma unroll
+  for (int j = 1; j < NK; ++j) fl = li == j ? f[j] : fl;
+  SweepIXA<R> nxt{};
+  if (li < NK && tend > 0) nxt = ixa[(long)fl * P.ntau];
 
   for (int t0 = 0; t0 < tend; t0 += FOLD) {
     const int t1 = min(tend, t0 + FOLD);
@@ -338,10 +352,14 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
       R a[NK];
 #pragma unroll
       for (int j = 0; j < NK; ++j) {
-        const SweepIXA<R> v = ixa[(long)f[j] * P.ntau + t];
-        ix[j] = v.ix;
-        a[j] = v.a;
+        ix[j] = sweep_lane(nxt.ix, j);
+        a[j] = sweep_lane(nxt.a, j);
       }
+      if (li < NK && t + 1 < tend) nxt = ixa[(long)fl * P.ntau + t + 1];
+      R src[NK];
+#pragma unroll
+      for (int j = 0; j < NK; ++j)
+        src[j] = li < 3 * NL ? sf[(((long)f[j] * P.ntau + t) * P.m_size + mi) * 3 * NL + li] : (R)0;
       // the NK elements in turn; the J rows are loaded again only when an
       // element's index_x differs from the previous one's (neighbouring k3
       // usually share it), so one set of row registers serves the tile
@@ -359,12 +377,11 @@ __device__ __forceinline__ void sweep_item(const los_params &P, int xcd, int q, 
             }
           }
         }
-        // S x dtau of this element and step: [field T, E, B][L]
-        const R *sp = sf + (((long)f[j] * P.ntau + t) * P.m_size + mi) * 3 * NL;
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
           if (m > L) continue;
-          const R sT = sp[L], sE = sp[NL + L], sB = sp[2 * NL + L];
+          const R sT = sweep_lane(src[j], L), sE = sweep_lane(src[j], NL + L),
+                  sB = sweep_lane(src[j], 2 * NL + L);
           if (doT && L <= P.LmaxT) {
             R Jv = tv[L].j1 + a[j] * (tv[L].j0 - tv[L].j1);
             if (!PAIRS) Jv = ix[j] >= xmT[L] ? Jv : (R)0;
