@@ -182,6 +182,13 @@ int los_transfer2_prepare(struct precision *ppr, struct precision2 *ppr2, struct
                           struct bessels *pbs, struct bessels2 *pbs2, struct transfers *ptr,
                           struct transfers2 *ptr2);
 
+/* Paths of the sources files, "<ppt2->sources_dir>/sources_%03d.dat" per k1
+   (source/perturbations2.c:1051-1064: the set-up SONG's perturb2_init does when
+   sources are stored to or loaded from disk), allocated on the first call;
+   los_sources_free_paths releases them. */
+int los_sources_set_paths(struct perturbs2 *ppt2);
+void los_sources_free_paths(struct perturbs2 *ppt2);
+
 /* Process-wide default form used by transfer2_init (default LOS_FORM_FP64),
    and the device it runs on (default 0). */
 int los_set_default_form(int form);
@@ -254,8 +261,9 @@ void los_scenario_structs(los_scenario *s, struct precision **ppr, struct precis
 long los_scenario_n_transfers(los_scenario *s);
 int los_scenario_gather(los_scenario *s, double *out, long n, char *errmsg);
 
-/* Set a named scalar of the structures (e.g. "store_transfers_to_disk",
-   "transfers_dir" is set with los_scenario_set_string). */
+/* Set a named scalar of the structures (e.g. "store_transfers_to_disk";
+   "transfers_dir" and "sources_dir" are set with los_scenario_set_string, the
+   latter also setting the sources file paths). */
 int los_scenario_set(los_scenario *s, const char *key, double value, char *errmsg);
 int los_scenario_set_string(los_scenario *s, const char *key, const char *value,
                             char *errmsg);
@@ -273,6 +281,10 @@ int los_scenario_override_J(los_scenario *s, int index_J, int index_L, int index
                             int index_m, int xmin, const double *values, int n, char *errmsg);
 int los_scenario_scatter(los_scenario *s, const double *in, long n, char *errmsg);
 int los_scenario_k3_sizes(los_scenario *s, int *out, long n, char *errmsg);
+/* Write every allocated k1 level of the sources to its sources file and free
+   it, as SONG's perturbations2 stage does with store_sources_to_disk; needs
+   "sources_dir" set. */
+int los_scenario_store_sources(los_scenario *s, char *errmsg);
 double *los_scenario_source_ptr(los_scenario *s, int index_tp, int index_k1, int index_k2);
 double *los_scenario_J_row(los_scenario *s, int index_J, int index_L, int index_l, int index_m,
                            int *xmin, int *xsize);

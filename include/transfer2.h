@@ -140,6 +140,10 @@ int transfer2_free_k1_level(struct perturbs2 *ppt2, struct transfers2 *ptr2,
 int perturb2_allocate_k1_level(struct perturbs2 *ppt2, int index_k1);
 int perturb2_load_sources_from_disk(struct perturbs2 *ppt2, int index_k1);
 int perturb2_free_k1_level(struct perturbs2 *ppt2, int index_k1);
+/* source/perturbations2.c:15347-15414 (perturb2_store_sources_to_disk): write
+   the k1 level of ppt2->sources to ppt2->sources_paths[index_k1], the file
+   perturb2_load_sources_from_disk reads back. Weak as the three above. */
+int perturb2_store_sources_to_disk(struct perturbs2 *ppt2, int index_k1);
 
 #ifdef __cplusplus
 }

@@ -427,6 +427,8 @@ int los_scenario_set(los_scenario *s, const char *key, double v, char *err)
 {
   if (!strcmp(key, "store_transfers_to_disk")) s->pr2.store_transfers_to_disk = (short)v;
   else if (!strcmp(key, "load_transfers_from_disk")) s->pr2.load_transfers_from_disk = (short)v;
+  else if (!strcmp(key, "store_sources_to_disk")) s->pr2.store_sources_to_disk = (short)v;
+  else if (!strcmp(key, "load_sources_from_disk")) s->pr2.load_sources_from_disk = (short)v;
   else if (!strcmp(key, "stop_at_transfers2")) s->tr2.stop_at_transfers2 = (short)v;
   else if (!strcmp(key, "has_perturbations2")) s->pt2.has_perturbations2 = (short)v;
   else if (!strcmp(key, "has_cmb_bispectra")) s->pt2.has_cmb_bispectra = (short)v;
@@ -453,6 +455,11 @@ int los_scenario_set_string(los_scenario *s, const char *key, const char *value,
 {
   if (!strcmp(key, "transfers_dir")) {
     snprintf(s->tr2.transfers_dir, _FILENAMESIZE_, "%s", value);
+    return _SUCCESS_;
+  }
+  if (!strcmp(key, "sources_dir")) {
+    snprintf(s->pt2.sources_dir, _FILENAMESIZE_, "%s", value);
+    los_call(los_sources_set_paths(&s->pt2), s->pt2.error_message, err);
     return _SUCCESS_;
   }
   los_fail(err, "los_scenario_set_string: unknown key '%s'", key);
@@ -571,10 +578,23 @@ int los_scenario_k3_sizes(los_scenario *s, int *out, long n, char *err)
   return _SUCCESS_;
 }
 
+int los_scenario_store_sources(los_scenario *s, char *err)
+{
+  struct perturbs2 *ppt2 = &s->pt2;
+  los_test(ppt2->sources_paths == NULL, err, "set sources_dir first");
+  for (int i1 = 0; i1 < ppt2->k_size; ++i1) {
+    if (!ppt2->has_allocated_sources[i1]) continue;
+    los_call(perturb2_store_sources_to_disk(ppt2, i1), ppt2->error_message, err);
+    los_call(perturb2_free_k1_level(ppt2, i1), ppt2->error_message, err);
+  }
+  return _SUCCESS_;
+}
+
 void los_scenario_destroy(los_scenario *s)
 {
   if (!s) return;
   struct perturbs2 *ppt2 = &s->pt2;
+  los_sources_free_paths(ppt2);
   for (int tp = 0; tp < ppt2->tp2_size; ++tp) {
     for (int i1 = 0; i1 < ppt2->k_size; ++i1) {
       if (ppt2->has_allocated_sources[i1] && ppt2->sources[tp][i1]) {

@@ -79,6 +79,55 @@ __attribute__((weak)) int perturb2_load_sources_from_disk(struct perturbs2 *ppt2
   return _SUCCESS_;
 }
 
+/* source/perturbations2.c:15347-15414: one file per k1, for each type and each
+   k2 <= k1 the tau_size * k3_size doubles of that (k3, tau) level as they lie in
+   memory ([tau][k3], k3 fastest); levels with no k3 write nothing. */
+__attribute__((weak)) int perturb2_store_sources_to_disk(struct perturbs2 *ppt2, int index_k1)
+{
+  los_test(ppt2->sources_paths == NULL, ppt2->error_message, "no sources paths set");
+  los_test(ppt2->has_allocated_sources[index_k1] == _FALSE_, ppt2->error_message,
+           "the index_k1=%d level of ppt2->sources is not allocated", index_k1);
+  FILE *fp = fopen(ppt2->sources_paths[index_k1], "wb");
+  los_test(fp == NULL, ppt2->error_message, "could not open %s", ppt2->sources_paths[index_k1]);
+  for (int tp = 0; tp < ppt2->tp2_size; ++tp)
+    for (int i2 = 0; i2 <= index_k1; ++i2) {
+      long n = (long)ppt2->tau_size * ppt2->k3_size[index_k1][i2];
+      if (n > 0 && (long)fwrite(ppt2->sources[tp][index_k1][i2], sizeof(double), n, fp) != n) {
+        fclose(fp);
+        los_fail(ppt2->error_message, "error writing %s", ppt2->sources_paths[index_k1]);
+      }
+    }
+  los_test(fclose(fp) != 0, ppt2->error_message, "error closing %s",
+           ppt2->sources_paths[index_k1]);
+  return _SUCCESS_;
+}
+
+/* source/perturbations2.c:1051-1064: the path of each k1's sources file,
+   "<sources_dir>/sources_%03d.dat". */
+int los_sources_set_paths(struct perturbs2 *ppt2)
+{
+  if (ppt2->sources_paths == NULL) {
+    los_calloc(ppt2->sources_files, ppt2->k_size, sizeof(FILE *), ppt2->error_message);
+    los_calloc(ppt2->sources_paths, ppt2->k_size, sizeof(char *), ppt2->error_message);
+    for (int i1 = 0; i1 < ppt2->k_size; ++i1)
+      los_alloc(ppt2->sources_paths[i1], _FILENAMESIZE_, ppt2->error_message);
+  }
+  for (int i1 = 0; i1 < ppt2->k_size; ++i1)
+    snprintf(ppt2->sources_paths[i1], _FILENAMESIZE_, "%s/sources_%03d.dat", ppt2->sources_dir,
+             i1);
+  return _SUCCESS_;
+}
+
+void los_sources_free_paths(struct perturbs2 *ppt2)
+{
+  if (ppt2->sources_paths)
+    for (int i1 = 0; i1 < ppt2->k_size; ++i1) free(ppt2->sources_paths[i1]);
+  free(ppt2->sources_paths);
+  free(ppt2->sources_files);
+  ppt2->sources_paths = NULL;
+  ppt2->sources_files = NULL;
+}
+
 __attribute__((weak)) int perturb2_free_k1_level(struct perturbs2 *ppt2, int index_k1)
 {
   los_test(ppt2->has_allocated_sources[index_k1] == _FALSE_, ppt2->error_message,

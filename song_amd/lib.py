@@ -77,6 +77,7 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_scenario_override_J": (i, [vp, i, i, i, i, i, C.POINTER(d), i, ch]),
         "los_scenario_scatter": (i, [vp, C.POINTER(d), C.c_long, ch]),
         "los_scenario_k3_sizes": (i, [vp, C.POINTER(i), C.c_long, ch]),
+        "los_scenario_store_sources": (i, [vp, ch]),
         "los_bessel2_build": (i, [vp] * 5 + [ch]),
         "los_bessel2_build_gpu": (i, [vp] * 5 + [i, ch]),
         "los_scenario_rebuild_J": (i, [vp, i, i, C.POINTER(d), ch]),
@@ -307,6 +308,11 @@ class Scenario:
         check(self.lib.los_scenario_k3_sizes(self.ptr, out.ctypes.data_as(C.POINTER(C.c_int)),
                                              out.size, err), err, "k3_sizes")
         return out.reshape(-1, 3)
+
+    def store_sources(self) -> None:
+        """Write every k1 level of the sources to its sources file and free it."""
+        err = _err()
+        check(self.lib.los_scenario_store_sources(self.ptr, err), err, "store_sources")
 
     def override_k(self, k) -> None:
         v = np.ascontiguousarray(k, dtype=np.float64)

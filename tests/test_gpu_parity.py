@@ -133,3 +133,21 @@ def test_fast_forms_within_tolerance(form, tol_field, tol_row, oracle):
     assert row.max() < tol_row
     s.close()
     ref_s.close()
+
+
+@pytest.mark.gpu
+def test_fp64_bit_exact_sources_from_disk(oracle, tmp_path):
+    """transfer2_init with load_sources_from_disk (source/transfer2.c:329-343): every k1
+    level is read from SONG's sources_%03d.dat files that an earlier stage wrote and freed;
+    the result equals the oracle's on the in-memory sources bit for bit."""
+    cfg = L.Config(**CASES["TEB_m012"])
+    print("case sources_from_disk", flush=True)
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    s = L.Scenario(cfg)
+    s.set_string("sources_dir", str(tmp_path))
+    s.store_sources()
+    s.set("load_sources_from_disk", 1)
+    s.transfer2_init()
+    got = s.gather()
+    assert np.array_equal(got, ref)
+    s.close()

@@ -6,7 +6,10 @@
   (source/transfer2.c:99-110, :151-157, :1408-1411, :1764-1771);
 * transfers written by the oracle in SONG's file format (one raw FP64 file per
   transfer type, source/transfer2.c:2929-2963) load through the library's
-  transfer2_load_transfers_from_disk (:705-763).
+  transfer2_load_transfers_from_disk (:705-763);
+* sources written in SONG's sources_%03d.dat format
+  (source/perturbations2.c:15347-15414, paths :1051-1064) hold the in-memory
+  blocks in SONG's order and load back bit for bit (:521-602).
 """
 import ctypes as C
 import os
@@ -141,3 +144,60 @@ def test_engine_rejects_unsupported_options():
         assert msg in str(exc.value)
         s._inited = False
         s.close()
+
+
+def _tp2_size(cfg):
+    """Source types: (L,m) with m <= L per field, T then E then B (B when m_max > 0)."""
+    def n_lm(l_max):
+        return sum(1 for L_ in range(l_max + 1) for m in cfg.m_list if m <= L_)
+    n = n_lm(cfg.l_max_los_t) if cfg.has_T else 0
+    if cfg.has_E:
+        n += n_lm(cfg.l_max_los_p) * (2 if max(cfg.m_list) > 0 else 1)
+    return n
+
+
+def test_sources_files_round_trip(tmp_path):
+    cfg = L.Config(**SMALL)
+    tp2 = _tp2_size(cfg)
+    assert tp2 == 12  # T, E, B x (L,m) in {(0,0), (1,0), (2,0), (2,2)}
+    ref = L.Scenario(cfg)  # same seed: the sources the writer holds
+    w = L.Scenario(cfg)
+    with pytest.raises(L.LosError, match="sources_dir"):
+        w.store_sources()
+    w.set_string("sources_dir", str(tmp_path))
+    w.store_sources()  # writes and frees every k1 level
+    with pytest.raises(L.LosError):
+        w.source(0, 0, 0)
+    w.close()
+    files = sorted(os.listdir(tmp_path))
+    assert files == ["sources_%03d.dat" % i for i in range(cfg.k_size)]
+    # file layout: types outermost, then k2 <= k1, then the [tau][k3] block
+    for i1 in range(cfg.k_size):
+        raw = np.fromfile(tmp_path / files[i1], dtype="<f8")
+        want = np.concatenate([ref.source(tp, i1, i2).ravel()
+                               for tp in range(tp2) for i2 in range(i1 + 1)])
+        assert raw.size == tp2 * cfg.tau_size * cfg.k3_size * (i1 + 1)
+        assert np.array_equal(raw, want)
+    # the library's perturb2_load_sources_from_disk reads them back bit for bit
+    lib = L.load()
+    r = L.Scenario(cfg)
+    r.set_string("sources_dir", str(tmp_path))
+    ppt2 = r.s["ppt2"]
+    for i1 in range(cfg.k_size):
+        assert lib.perturb2_free_k1_level(ppt2, i1) == 0
+        assert lib.perturb2_allocate_k1_level(ppt2, i1) == 0
+        assert not np.any(r.source(0, i1, 0))
+        assert lib.perturb2_load_sources_from_disk(ppt2, i1) == 0
+        for tp in range(tp2):
+            for i2 in range(i1 + 1):
+                assert np.array_equal(r.source(tp, i1, i2), ref.source(tp, i1, i2))
+    # a truncated file is an error, not a short read
+    last = tmp_path / files[-1]
+    data = last.read_bytes()
+    last.write_bytes(data[:-8])
+    i1 = cfg.k_size - 1
+    assert lib.perturb2_free_k1_level(ppt2, i1) == 0
+    assert lib.perturb2_allocate_k1_level(ppt2, i1) == 0
+    assert lib.perturb2_load_sources_from_disk(ppt2, i1) == 1
+    r.close()
+    ref.close()
