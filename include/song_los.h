@@ -314,6 +314,46 @@ void los_sph_bessel_array(int lmax, double x, double *out);
 int los_wigner3j_j1(int j2, int j3, int m2, int m3, int *j1min, int *j1max, double *out,
                     int out_cap);
 
+/* ------------------------------------------------------------------------
+ * k3 integral of the transfer functions (the consumer of this stage in the
+ * bispectrum module): the loop over (l3, k1, k2, r) of
+ * bispectra2_intrinsic_integrate_over_k3 (source/bispectra2.c:1182-1410) with
+ * bessel2_convolution (source/bessel2.c:499-605, linear interpolation, g = 1)
+ * on HIP device `device`, plain FP64 with the reference's operation order:
+ *   I[q][r][p] = 2 * (coef * (0.5 * sum_k3 k3*k3*j_L(k3 r)*T_qp(k3) * dk3)).
+ * Row q is one transfer row (SONG: tt = index_tt2_k3 + lm_array[l3][M3]) over
+ * every pair; index_L[q] is the j table of its L3 (-1: the reference skips the
+ * row, which stays zero); j tables are j_l1 from index_xmin_l1 on, with their
+ * x_min_l1 (bessel2.h:95-106). Inputs are uploaded once by _create and stay
+ * resident; _run launches the kernel `reps` times and gives the HIP-event
+ * time per launch; _result copies I back in [q][r][p] order.
+ * ---------------------------------------------------------------------- */
+typedef struct los_k3int_desc {
+  int n_pairs;             /* (k1,k2) pairs */
+  const long *pair_off;    /* n_pairs + 1 offsets of each pair's k3 points */
+  const double *k3;        /* concatenated k3 grids (transfer2_get_k3_list) */
+  int n_q;                 /* transfer rows */
+  const double *transfer;  /* [q][pair_off[n_pairs]] */
+  const int *index_L;      /* per row: j table of L3, or -1 */
+  int n_j;                 /* j tables */
+  const long *j_off;       /* n_j + 1 offsets into j */
+  const double *j;         /* j_l1[t][x - index_xmin_l1[t]] */
+  const double *x_min;     /* x_min_l1[t] */
+  double xx_step;          /* x grid step */
+  int n_r;                 /* r grid */
+  const double *r;
+  double coef;             /* M3 coefficient (1 for M3 = 0) */
+} los_k3int_desc;
+
+typedef struct los_k3int los_k3int;
+int los_k3int_create(const los_k3int_desc *desc, int device, los_k3int **out, char *errmsg);
+int los_k3int_run(los_k3int *h, int reps, float *ms, char *errmsg);
+int los_k3int_result(los_k3int *h, double *out, long n, char *errmsg);
+/* out[0] live terms (T != 0 and k3 r >= x_min), out[1] algorithmic bytes per
+   launch (inputs once, the j windows reached once, outputs once). */
+int los_k3int_counts(los_k3int *h, double *out, int n);
+void los_k3int_destroy(los_k3int *h);
+
 #ifdef __cplusplus
 }
 #endif
