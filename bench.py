@@ -12,6 +12,9 @@ Workloads (seeded synthetic sources: A ~ N(0,1) per (field,L,m,k1,k2) x (k3/0.01
       run from 8 to about 400 per pair; T+E(+B), m = 0,1,2, L = 0..6, 96 l in [2, 2000];
       every pair in one batch, the run captured in one hipGraph, tile shapes autotuned.
   c4  c2 extended to l_max = 3000 with 128 l values (precision study of the FP32 form).
+  k3int  the bispectrum module's k3 integral of the c2 transfers: the T rows of every l3
+      (M3 = 0) over the 528 pairs' fine k3 grids, against j_l3(k3 r) on 200 r in
+      [13600, 14400] Mpc (x step 0.3), plain FP64, bit-identical to the CPU code.
 
 A step is one pass of the LOS stage over every pair of the workload (source re-layout,
 k3 spline, LOS integral) with the source and projection tables already resident in HBM.
@@ -51,6 +54,7 @@ WORKLOADS = {
     # serving: the pairs of the c3 grid with l up to 3000 (jobs pick l_max prefixes)
     "c5": dict(_C2, k_size=64, k_log=2, k_max=0.2, k3_size=8, dk_fine=3.5e-4, xx_max=6000.0,
                l_size=128, l_max=3000),
+    "k3int": _C2,
     # small workload for quick checks
     "tiny": dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=4, l_max_los_p=4, k_size=6,
                  k3_size=16, tau_size=128, l_size=12, l_max=400, dk_fine=1e-3, xx_max=1500.0),
@@ -67,6 +71,8 @@ DESCR = {
     "c5": ("serving: Poisson jobs of 1-64 pairs of the c3 grid, l_max in {500,1000,2000,3000}, "
            "T+E+B m=0,1,2, resident tables, HBM-budget admission, 3 slots"),
     "tiny": "small check workload",
+    "k3int": ("k3 integral of the c2 T transfers: 96 l3 (M3=0) x 200 r in [13600,14400] x 528 "
+              "pairs, 41,759 fine k3, j_l3 on x step 0.3, plain FP64"),
 }
 
 FORMS = {"fp64": L.FORM_FP64, "fp64fast": L.FORM_FP64_FAST, "fp32c": L.FORM_FP32C}
@@ -286,6 +292,110 @@ def serve_bench(args, cfg):
     return 0
 
 
+def k3int_bench(args, cfg):
+    """The k3 integral of the transfers (bispectra2.c:1182-1410) on the c2 transfers."""
+    lib = L.load()
+    t0 = time.time()
+    scen = L.Scenario(cfg)
+    scen.transfer2_init()  # the LOS stage on the GPU gives the transfers
+    flat = scen.gather()
+    sizes = scen.k3_sizes()[:, 0].astype(np.int64)
+    pair_off = np.concatenate([[0], np.cumsum(sizes)])
+    n_k3 = int(pair_off[-1])
+    k3 = np.empty(n_k3)
+    st = scen.s
+    q = 0
+    for i1 in range(cfg.k_size):
+        for i2 in range(i1 + 1):
+            buf = np.zeros(int(sizes[q]) + 4)
+            last = C.c_int()
+            rc = lib.transfer2_get_k3_list(st["ppr"], st["ppr2"], st["ppt2"], st["pbs"],
+                                           st["pbs2"], st["ptr2"], i1, i2,
+                                           buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                           C.byref(last))
+            if rc != 0:
+                raise RuntimeError("transfer2_get_k3_list failed: " + scen.error_message())
+            k3[pair_off[q]:pair_off[q + 1]] = buf[:sizes[q]]
+            q += 1
+    n_l = cfg.l_size
+    T = np.ascontiguousarray(flat[: n_l * n_k3].reshape(n_l, n_k3))  # T field, m = 0 rows
+    ls = [int(v) for v in scen.grids()[0][:n_l]]
+    scen.close()
+    r = np.linspace(13600.0, 14400.0, 200)
+    step = cfg.xx_step
+    nx = int(k3.max() * r.max() / step) + 3
+    lmax = max(ls)
+    jl = np.empty((nx, lmax + 1))
+    row = np.empty(lmax + 1)
+    for ix in range(nx):
+        lib.los_sph_bessel_array(lmax, ix * step, row.ctypes.data_as(C.POINTER(C.c_double)))
+        jl[ix] = row
+    j_off, vals, x_min = [0], [], []
+    for l in ls:
+        col = jl[:, l]
+        i0 = 0 if l == 0 else max(int(np.argmax(np.abs(col) >= 1e-12)) - 1, 0)
+        vals.append(col[i0:])
+        x_min.append(i0 * step)
+        j_off.append(j_off[-1] + nx - i0)
+    j = np.concatenate(vals)
+    print(f"# k3int inputs ({n_l} rows, {len(sizes)} pairs, {n_k3} k3, {r.size} r) in "
+          f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    h = L.K3Integral(pair_off, k3, T, np.arange(n_l), j_off, j, x_min, step, r, 1.0)
+    for _ in range(max(args.warmup, 1)):
+        h.run(1)
+    ms = h.run(args.steps)
+    got = h.result()
+    cnt = h.counts()
+    h.close()
+    n_val = got.size
+    flop = 16.0 * cnt["live_terms"]
+    achieved = cnt["alg_bytes"] / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "k3-integral values I_l3(r,k1,k2)/s", "value": n_val / (ms * 1e-3),
+        "unit": "values/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (c2 transfers from the LOS stage on the GPU)",
+        "config": {"workload": f"k3int: {DESCR['k3int']}", "values": n_val,
+                   "live_terms": cnt["live_terms"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "los_k3int_kernel", "kernel_ms": ms,
+                     "alg_bytes": cnt["alg_bytes"],
+                     "valu": {"achieved": flop / (ms * 1e-3) / 1e12, "peak": 78.6,
+                              "unit": "TFLOP/s", "frac": flop / (ms * 1e-3) / 1e12 / 78.6,
+                              "flop_alg": flop}},
+    }
+    if not args.no_cpu:
+        o = C.CDLL(os.path.join(ROOT, "oracle", "liblos_oracle.so"))
+        f = o.los_ref_k3_integral
+        dp, ip, lp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+        f.argtypes = [C.c_int, lp, dp, C.c_int, dp, ip, lp, dp, dp, C.c_double, C.c_int, dp,
+                      C.c_int, C.c_double, dp]
+        po = np.ascontiguousarray(pair_off, dtype=np.int64)
+        jo = np.ascontiguousarray(j_off, dtype=np.int64)
+        xm = np.ascontiguousarray(x_min)
+        done, tcpu = 0, 0.0
+        while done < n_l and tcpu < args.cpu_seconds:
+            il = np.array([done], dtype=np.int32)
+            out = np.zeros((1, r.size, len(sizes)))
+            t1 = time.perf_counter()
+            f(len(sizes), po.ctypes.data_as(lp), k3.ctypes.data_as(dp), 1,
+              T[done].ctypes.data_as(dp), il.ctypes.data_as(ip), jo.ctypes.data_as(lp),
+              j.ctypes.data_as(dp), xm.ctypes.data_as(dp), step, r.size, r.ctypes.data_as(dp),
+              0, 1.0, out.ctypes.data_as(dp))
+            tcpu += time.perf_counter() - t1
+            if not np.array_equal(out[0], got[done]):
+                raise RuntimeError(f"k3int: GPU row {done} differs from the CPU code")
+            done += 1
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        line["cpu_baseline"] = {"value": done * r.size * len(sizes) / tcpu, "unit": "values/s",
+                                "cores": threads, "kind": "port",
+                                "sample": f"{done} of {n_l} l3 rows ({tcpu:.1f} s, OpenMP "
+                                          "dynamic over pairs); each equal to the GPU's"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +431,8 @@ def main(argv=None):
     cfg = L.Config(**WORKLOADS[args.workload])
     if args.workload == "c5":
         return serve_bench(args, cfg)
+    if args.workload == "k3int":
+        return k3int_bench(args, cfg)
     t0 = time.time()
     scen = L.Scenario(cfg)
     print(f"# inputs built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)

@@ -16,6 +16,9 @@
  *   time grid                 source/transfer2.c:2366-2585
  *   interpolate in k / time   source/transfer2.c:2612-2763, :2783-2908
  *   store / alloc / free      source/transfer2.c:636-691, :769-838, :2929-3104
+ * and the bispectrum module's k3 integral of the transfers (los_ref_k3_integral):
+ *   integrate_over_k3 loop    source/bispectra2.c:1182-1410
+ *   bessel2_convolution       source/bessel2.c:499-605 (linear, g = NULL)
  * The reference cannot be compiled here (its CLASS submodule is empty), so
  * the CLASS helpers it calls are restated from their documented behaviour:
  *   array_spline_table_columns(..., _SPLINE_EST_DERIV_): clamped cubic spline,
@@ -1064,4 +1067,71 @@ int los_ref_size_indexl_indexm(const int *lv, int ls, const int *mv, int ms)
 int los_ref_offset_indexl_indexm(int l, int m, const int *lv, int ls, const int *mv, int ms)
 {
   return ref_offset_indexl_indexm(l, m, lv, ls, mv, ms);
+}
+
+/* ---------------------------------------------------------------------------
+ * k3 integral of the transfer functions, source/bispectra2.c:1182-1410: for
+ * each transfer row q (one l3), pair p and r, bessel2_convolution
+ * (source/bessel2.c:499-605) over the pair's k3 grid with its trapezoid
+ * measure (:1283-1287), then the M3 coefficient (only for M3 != 0, :1355-1356)
+ * and the factor 2 (:1361). Rows with index_L < 0 are the skipped (l3, M3)
+ * configurations (:1196-1205) and stay zero. out[(q * n_r + r) * n_pairs + p].
+ * ------------------------------------------------------------------------- */
+static double ref_bessel2_convolution(const double *kk, const double *dkk, int k_size,
+                                      const double *f, const double *j, double x_min,
+                                      double xx_step, double r)
+{
+  double integral = 0;
+  for (int ik = 0; ik < k_size; ++ik) {
+    double f_in_k = f[ik];
+    if (f_in_k == 0.) continue;
+    double g_in_k = 1.;
+    double k = kk[ik];
+    double x = k * r;
+    if (x < x_min) continue;
+    int index_x = (int)((x - x_min) / xx_step);
+    double a = (x_min + xx_step * (index_x + 1) - x) / xx_step;
+    double jv = a * j[index_x] + (1. - a) * j[index_x + 1];
+    double integrand = k * k * jv * f_in_k * g_in_k;
+    integral += integrand * dkk[ik];
+  }
+  integral *= 0.5;
+  return integral;
+}
+
+int los_ref_k3_integral(int n_pairs, const long *pair_off, const double *k3, int n_q,
+                        const double *transfer, const int *index_L, const long *j_off,
+                        const double *j, const double *x_min, double xx_step, int n_r,
+                        const double *r, int abs_M3, double M3_coefficient, double *out)
+{
+  const long n_k3 = n_pairs > 0 ? pair_off[n_pairs] : 0;
+  double *dk = malloc((size_t)(n_k3 > 0 ? n_k3 : 1) * sizeof(double));
+  if (!dk) return 1;
+  for (int p = 0; p < n_pairs; ++p) {
+    const long b = pair_off[p];
+    const int n = (int)(pair_off[p + 1] - b);
+    if (n < 2) { free(dk); return 1; }
+    dk[b] = k3[b + 1] - k3[b];
+    for (int i = 1; i < n - 1; ++i) dk[b + i] = k3[b + i + 1] - k3[b + i - 1];
+    dk[b + n - 1] = k3[b + n - 1] - k3[b + n - 2];
+  }
+  memset(out, 0, sizeof(double) * (size_t)n_q * n_r * n_pairs);
+  for (int q = 0; q < n_q; ++q) {
+    const int iL = index_L[q];
+    if (iL < 0) continue;
+#pragma omp parallel for schedule(dynamic)
+    for (int p = 0; p < n_pairs; ++p) {
+      const long b = pair_off[p];
+      const int n = (int)(pair_off[p + 1] - b);
+      for (int ir = 0; ir < n_r; ++ir) {
+        double v = ref_bessel2_convolution(k3 + b, dk + b, n, transfer + (long)q * n_k3 + b,
+                                           j + j_off[iL], x_min[iL], xx_step, r[ir]);
+        if (abs_M3 != 0) v *= M3_coefficient;
+        v *= 2;
+        out[((long)q * n_r + ir) * n_pairs + p] = v;
+      }
+    }
+  }
+  free(dk);
+  return 0;
 }

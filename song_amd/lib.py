@@ -58,6 +58,16 @@ class CConfig(C.Structure):
     ]
 
 
+class CK3IntDesc(C.Structure):
+    """Mirror of ``los_k3int_desc`` (include/song_los.h)."""
+    _fields_ = [("n_pairs", C.c_int), ("pair_off", C.POINTER(C.c_long)),
+                ("k3", C.POINTER(C.c_double)), ("n_q", C.c_int),
+                ("transfer", C.POINTER(C.c_double)), ("index_L", C.POINTER(C.c_int)),
+                ("n_j", C.c_int), ("j_off", C.POINTER(C.c_long)), ("j", C.POINTER(C.c_double)),
+                ("x_min", C.POINTER(C.c_double)), ("xx_step", C.c_double), ("n_r", C.c_int),
+                ("r", C.POINTER(C.c_double)), ("coef", C.c_double)]
+
+
 def _declare(lib: C.CDLL) -> C.CDLL:
     vp, i, d, ch = C.c_void_p, C.c_int, C.c_double, C.c_char_p
     sig = {
@@ -117,6 +127,11 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_copy_probe": (i, [vp, d, C.POINTER(C.c_float), ch]),
         "los_engine_alg_bytes": (d, [vp]),
         "los_engine_stream": (vp, [vp]),
+        "los_k3int_create": (i, [C.POINTER(CK3IntDesc), i, C.POINTER(vp), ch]),
+        "los_k3int_run": (i, [vp, i, C.POINTER(C.c_float), ch]),
+        "los_k3int_result": (i, [vp, C.POINTER(d), C.c_long, ch]),
+        "los_k3int_counts": (i, [vp, C.POINTER(d), i]),
+        "los_k3int_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -529,6 +544,70 @@ class Server:
     def close(self) -> None:
         if self.ptr:
             self.lib.los_server_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class K3Integral:
+    """k3 integral of transfer rows against j_L(k3 r) on the GPU (los_k3int_*).
+
+    pair_off (n_pairs+1), k3 (concatenated grids), transfer [n_q][n_k3], index_L (n_q, -1 =
+    skipped row), j_off (n_j+1) / j / x_min (j tables from their first stored point), r,
+    coef (M3 coefficient). Inputs are uploaded at construction and stay resident.
+    """
+
+    def __init__(self, pair_off, k3, transfer, index_L, j_off, j, x_min, xx_step, r,
+                 coef=1.0, device: int = 0):
+        self.lib = load()
+        self._keep = [np.ascontiguousarray(pair_off, dtype=np.int64),
+                      np.ascontiguousarray(k3, dtype=np.float64),
+                      np.ascontiguousarray(transfer, dtype=np.float64),
+                      np.ascontiguousarray(index_L, dtype=np.int32),
+                      np.ascontiguousarray(j_off, dtype=np.int64),
+                      np.ascontiguousarray(j, dtype=np.float64),
+                      np.ascontiguousarray(x_min, dtype=np.float64),
+                      np.ascontiguousarray(r, dtype=np.float64)]
+        po, k, t, il, jo, jj, xm, rr = self._keep
+        self.n_pairs, self.n_q, self.n_r = po.size - 1, il.size, rr.size
+        assert t.size == self.n_q * (po[-1] if po.size else 0), "transfer must be [n_q][n_k3]"
+        dp, ip, lp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+        self.desc = CK3IntDesc(self.n_pairs, po.ctypes.data_as(lp), k.ctypes.data_as(dp),
+                               self.n_q, t.ctypes.data_as(dp), il.ctypes.data_as(ip),
+                               xm.size, jo.ctypes.data_as(lp), jj.ctypes.data_as(dp),
+                               xm.ctypes.data_as(dp), float(xx_step), self.n_r,
+                               rr.ctypes.data_as(dp), float(coef))
+        self.ptr = C.c_void_p()
+        err = _err()
+        check(self.lib.los_k3int_create(C.byref(self.desc), device, C.byref(self.ptr), err),
+              err, "los_k3int_create")
+
+    def run(self, reps: int = 1) -> float:
+        """Launch the kernel `reps` times on the resident inputs; HIP-event ms per launch."""
+        ms = C.c_float()
+        err = _err()
+        check(self.lib.los_k3int_run(self.ptr, reps, C.byref(ms), err), err, "los_k3int_run")
+        return ms.value
+
+    def result(self) -> np.ndarray:
+        out = np.empty((self.n_q, self.n_r, self.n_pairs), dtype=np.float64)
+        err = _err()
+        check(self.lib.los_k3int_result(self.ptr, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                        out.size, err), err, "los_k3int_result")
+        return out
+
+    def counts(self):
+        v = (C.c_double * 2)()
+        self.lib.los_k3int_counts(self.ptr, v, 2)
+        return {"live_terms": v[0], "alg_bytes": v[1]}
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.los_k3int_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):
