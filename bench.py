@@ -15,6 +15,8 @@ Workloads (seeded synthetic sources: A ~ N(0,1) per (field,L,m,k1,k2) x (k3/0.01
   k3int  the bispectrum module's k3 integral of the c2 transfers: the T rows of every l3
       (M3 = 0) over the 528 pairs' fine k3 grids, against j_l3(k3 r) on 200 r in
       [13600, 14400] Mpc (x step 0.3), plain FP64, bit-identical to the CPU code.
+  cl2  the power-spectrum module's (k1, k2, k3) sum of the c2 transfers: C_l^TT and C_l^EE
+      for every l and m = 0, 1, 2 (rescaled sources), power-law primordial spectrum.
 
 A step is one pass of the LOS stage over every pair of the workload (source re-layout,
 k3 spline, LOS integral) with the source and projection tables already resident in HBM.
@@ -55,6 +57,7 @@ WORKLOADS = {
     "c5": dict(_C2, k_size=64, k_log=2, k_max=0.2, k3_size=8, dk_fine=3.5e-4, xx_max=6000.0,
                l_size=128, l_max=3000),
     "k3int": _C2,
+    "cl2": _C2,
     # small workload for quick checks
     "tiny": dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=4, l_max_los_p=4, k_size=6,
                  k3_size=16, tau_size=128, l_size=12, l_max=400, dk_fine=1e-3, xx_max=1500.0),
@@ -73,6 +76,8 @@ DESCR = {
     "tiny": "small check workload",
     "k3int": ("k3 integral of the c2 T transfers: 96 l3 (M3=0) x 200 r in [13600,14400] x 528 "
               "pairs, 41,759 fine k3, j_l3 on x step 0.3, plain FP64"),
+    "cl2": ("C_l^TT and C_l^EE of the c2 transfers: 96 l x m=0,1,2 x 528 pairs x 41,759 fine "
+            "k3, rescaled sources, power-law P(k), FP64"),
 }
 
 FORMS = {"fp64": L.FORM_FP64, "fp64fast": L.FORM_FP64_FAST, "fp32c": L.FORM_FP32C}
@@ -292,6 +297,101 @@ def serve_bench(args, cfg):
     return 0
 
 
+def _pair_k3_grids(lib, scen, cfg):
+    """Fine k3 grid of every pair (k1-major, k2 <= k1) from transfer2_get_k3_list."""
+    sizes = scen.k3_sizes()[:, 0].astype(np.int64)
+    pair_off = np.concatenate([[0], np.cumsum(sizes)])
+    k3 = np.empty(int(pair_off[-1]))
+    st = scen.s
+    q = 0
+    for i1 in range(cfg.k_size):
+        for i2 in range(i1 + 1):
+            buf = np.zeros(int(sizes[q]) + 4)
+            last = C.c_int()
+            rc = lib.transfer2_get_k3_list(st["ppr"], st["ppr2"], st["ppt2"], st["pbs"],
+                                           st["pbs2"], st["ptr2"], i1, i2,
+                                           buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                           C.byref(last))
+            if rc != 0:
+                raise RuntimeError("transfer2_get_k3_list failed: " + scen.error_message())
+            k3[pair_off[q]:pair_off[q + 1]] = buf[:sizes[q]]
+            q += 1
+    return sizes, pair_off, k3
+
+
+def cl2_bench(args, cfg):
+    """The second-order C_l sum (spectra2.c:241-590) over the c2 transfers."""
+    lib = L.load()
+    t0 = time.time()
+    scen = L.Scenario(cfg)
+    scen.transfer2_init()
+    flat = scen.gather()
+    sizes, pair_off, k3 = _pair_k3_grids(lib, scen, cfg)
+    n_k3 = int(pair_off[-1])
+    k = scen.grids()[1]
+    scen.close()
+    n_l, m = cfg.l_size, list(cfg.m_list)
+    n_lm = n_l * len(m)  # every l >= 2 carries m = 0, 1, 2
+    fields = flat.reshape(-1, n_k3)
+    T = fields[:n_lm].reshape(n_l, len(m), n_k3)
+    E = fields[n_lm:2 * n_lm].reshape(n_l, len(m), n_k3)
+    pk = 2.1e-9 * (k / 0.05) ** (0.965 - 1)
+    used = np.ones((n_l, len(m)), dtype=np.int32)
+    print(f"# cl2 inputs ({n_l} l x {len(m)} m, {len(sizes)} pairs, {n_k3} k3) in "
+          f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    res, ms_tot = {}, 0.0
+    for name, t in (("TT", T), ("EE", E)):
+        h = L.Cl2(k, pk, pair_off, k3, m, t, t, used, 1)
+        for _ in range(max(args.warmup, 1)):
+            h.run(1)
+        ms = h.run(args.steps)
+        res[name] = (h.result(), ms)
+        ms_tot += ms
+        h.close()
+    terms = 2 * n_l * len(m) * n_k3
+    # per term: 8 B of each of the two transfer rows (k3 grid and weights are shared by rows)
+    alg_bytes = 2 * (2 * 8.0 * n_l * len(m) * n_k3 + 8.0 * n_k3)
+    achieved = alg_bytes / (ms_tot * 1e-3) / 1e9
+    line = {
+        "metric": "second-order C_l (k1,k2,k3) terms/s", "value": terms / (ms_tot * 1e-3),
+        "unit": "terms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_tot, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (c2 transfers from the LOS stage on the GPU)",
+        "config": {"workload": f"cl2: {DESCR['cl2']}", "terms": terms,
+                   "ms_TT": res["TT"][1], "ms_EE": res["EE"][1]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "los_cl2_kernel",
+                     "kernel_ms": ms_tot / 2, "alg_bytes": alg_bytes / 2},
+    }
+    if not args.no_cpu:
+        o = C.CDLL(os.path.join(ROOT, "oracle", "liblos_oracle.so"))
+        f = o.los_ref_cl2
+        dp, ip, lp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+        f.argtypes = [C.c_int, dp, dp, lp, dp, C.c_int, C.c_int, ip, dp, dp, ip, C.c_int, dp, dp]
+        po = np.ascontiguousarray(pair_off, dtype=np.int64)
+        mm = np.array(m, dtype=np.int32)
+        worst, tcpu = 0.0, 0.0
+        for name, t in (("TT", T), ("EE", E)):
+            cl = np.zeros(n_l)
+            cl_m = np.zeros((n_l, len(m)))
+            t1 = time.perf_counter()
+            f(k.size, k.ctypes.data_as(dp), pk.ctypes.data_as(dp), po.ctypes.data_as(lp),
+              k3.ctypes.data_as(dp), n_l, len(m), mm.ctypes.data_as(ip), t.ctypes.data_as(dp),
+              t.ctypes.data_as(dp), used.ctypes.data_as(ip), 1, cl.ctypes.data_as(dp),
+              cl_m.ctypes.data_as(dp))
+            tcpu += time.perf_counter() - t1
+            worst = max(worst, float(np.max(np.abs(res[name][0] - cl_m) / np.abs(cl_m))))
+        line["max_rel_err_vs_cpu"] = worst
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        line["cpu_baseline"] = {"value": terms / tcpu, "unit": "terms/s", "cores": threads,
+                                "kind": "port",
+                                "sample": f"the whole workload ({tcpu:.2f} s, OpenMP dynamic "
+                                          "over l, sequential sums)"}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def k3int_bench(args, cfg):
     """The k3 integral of the transfers (bispectra2.c:1182-1410) on the c2 transfers."""
     lib = L.load()
@@ -433,6 +533,8 @@ def main(argv=None):
         return serve_bench(args, cfg)
     if args.workload == "k3int":
         return k3int_bench(args, cfg)
+    if args.workload == "cl2":
+        return cl2_bench(args, cfg)
     t0 = time.time()
     scen = L.Scenario(cfg)
     print(f"# inputs built in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)

@@ -354,6 +354,41 @@ int los_k3int_result(los_k3int *h, double *out, long n, char *errmsg);
 int los_k3int_counts(los_k3int *h, double *out, int n);
 void los_k3int_destroy(los_k3int *h);
 
+/* ------------------------------------------------------------------------
+ * Second-order C_l: the (k1, k2, k3) sum of spectra2_cls
+ * (source/spectra2.c:241-590, k3 loop :466-540) on HIP device `device`, FP64,
+ * for every l (n_l) and m (m_size) of the run:
+ *   cl_m[l][M] = sum_{k1, k2<=k1, k3} 2 k1 k2 k3 dk1 dk2 dk3 [scale^2]
+ *                * 2/pi/2/pi/2/pi/2/pi * P(k1) P(k2) * T1 T2
+ * with the reference's trapezoid half steps and triangle-edge corrections,
+ * P(k) = 2 pi^2 / k^3 pk[k] (pk: primordial_spectrum_at_k, linear) and
+ * scale^2 = (1 - cos^2)^m when rescale (ppt2->rescale_cmb_sources). Pairs are
+ * k1-major with k2 <= k1 over the whole k grid; transfer1/transfer2 hold the
+ * rows [l][M][k3 of all pairs] of the two fields of the spectrum (TT, EE, TE,
+ * BB); used[l][M] = 0 skips a row (BB with m = 0, :330-333). C_l itself is the
+ * sum over M. Results agree with the sequential sum to rounding.
+ * ---------------------------------------------------------------------- */
+typedef struct los_cl2_desc {
+  int k_size;               /* ppt2->k grid */
+  const double *k;
+  const double *pk;         /* primordial spectrum at each k */
+  const long *pair_off;     /* k_size (k_size+1)/2 + 1 offsets into k3 */
+  const double *k3;         /* concatenated k3 grids (transfer2_get_k3_list) */
+  int n_l;
+  int m_size;
+  const int *m;             /* the m values */
+  const double *transfer1;  /* [n_l][m_size][n_k3] */
+  const double *transfer2;  /* [n_l][m_size][n_k3] */
+  const int *used;          /* [n_l][m_size] */
+  int rescale;              /* ppt2->rescale_cmb_sources */
+} los_cl2_desc;
+
+typedef struct los_cl2 los_cl2;
+int los_cl2_create(const los_cl2_desc *desc, int device, los_cl2 **out, char *errmsg);
+int los_cl2_run(los_cl2 *h, int reps, float *ms, char *errmsg);
+int los_cl2_result(los_cl2 *h, double *cl_m, long n, char *errmsg);
+void los_cl2_destroy(los_cl2 *h);
+
 #ifdef __cplusplus
 }
 #endif

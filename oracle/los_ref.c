@@ -19,6 +19,8 @@
  * and the bispectrum module's k3 integral of the transfers (los_ref_k3_integral):
  *   integrate_over_k3 loop    source/bispectra2.c:1182-1410
  *   bessel2_convolution       source/bessel2.c:499-605 (linear, g = NULL)
+ * and the power-spectrum module's (k1,k2,k3) sum (los_ref_cl2):
+ *   spectra2_cls              source/spectra2.c:241-590 (k3 loop :466-540)
  * The reference cannot be compiled here (its CLASS submodule is empty), so
  * the CLASS helpers it calls are restated from their documented behaviour:
  *   array_spline_table_columns(..., _SPLINE_EST_DERIV_): clamped cubic spline,
@@ -1133,5 +1135,88 @@ int los_ref_k3_integral(int n_pairs, const long *pair_off, const double *k3, int
     }
   }
   free(dk);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Second-order C_l, source/spectra2.c:241-590: per l (OpenMP over l, :283),
+ * sequential sums over M, k1, k2 <= k1 and the pair's k3 of
+ * total_factor * T1 * T2 (:502-516) into cl[l] and cl_m[l][M]. k1/k2 half
+ * trapezoid steps :375-380 and :398-404, P(k) = 2 pi^2/k^3 pk :388, :420, the
+ * k3 steps with the triangle-edge corrections :486-500. Where the reference
+ * reads k3_grid[index_k3 + 1] at the last point (:495, one past the pair's
+ * grid, a value left from earlier pairs), the point past the end is taken as
+ * beyond k1 + k2. Rows with used[l][M] = 0 are skipped (BB with m = 0).
+ * ------------------------------------------------------------------------- */
+int los_ref_cl2(int k_size, const double *k, const double *pk, const long *pair_off,
+                const double *k3g, int n_l, int m_size, const int *mv, const double *t1,
+                const double *t2, const int *used, int rescale, double *cl, double *cl_m)
+{
+  const double PI = 3.1415926535897932384626433832795e0;
+  const int n_pairs = k_size * (k_size + 1) / 2;
+  const long n_k3 = pair_off[n_pairs];
+#pragma omp parallel for schedule(dynamic)
+  for (int il = 0; il < n_l; ++il) {
+    double result = 0;
+    for (int iM = 0; iM < m_size; ++iM) {
+      const int m = mv[iM];
+      double result_m = 0;
+      if (used[il * m_size + iM]) {
+        const double *T1 = t1 + ((long)il * m_size + iM) * n_k3;
+        const double *T2 = t2 + ((long)il * m_size + iM) * n_k3;
+        int p = 0;
+        for (int i1 = 0; i1 < k_size; ++i1) {
+          double step_k1;
+          if (i1 == 0) step_k1 = (k[1] - k[0]) / 2.;
+          else if (i1 == k_size - 1) step_k1 = (k[k_size - 1] - k[k_size - 2]) / 2.;
+          else step_k1 = (k[i1 + 1] - k[i1 - 1]) / 2.;
+          const double k1 = k[i1];
+          const double spectra_k1 = 2 * PI * PI / (k1 * k1 * k1) * pk[i1];
+          for (int i2 = 0; i2 <= i1; ++i2, ++p) {
+            double step_k2;
+            if (i1 == 0) step_k2 = 0.;
+            else if (i2 == 0) step_k2 = (k[1] - k[0]) / 2.;
+            else if (i2 == i1) step_k2 = (k[i1] - k[i1 - 1]) / 2.;
+            else step_k2 = (k[i2 + 1] - k[i2 - 1]) / 2.;
+            const double k2 = k[i2];
+            const double spectra_k2 = 2 * PI * PI / (k2 * k2 * k2) * pk[i2];
+            const long b = pair_off[p];
+            const int n = (int)(pair_off[p + 1] - b);
+            const double *g = k3g + b;
+            int triangular_first = 0, triangular_last = 0;
+            for (int i3 = 0; i3 < n; ++i3) {
+              const double k3 = g[i3];
+              double step_k3;
+              if (i3 == 0) step_k3 = (g[1] - g[0]) / 2.;
+              else if (i3 == n - 1) step_k3 = (g[n - 1] - g[n - 2]) / 2.;
+              else step_k3 = (g[i3 + 1] - g[i3 - 1]) / 2.;
+              if (k3 < k1 - k2 || k3 > k1 + k2) step_k3 = 0.;
+              if (k3 >= k1 - k2 && triangular_first == 0 && i3 < n - 1) {
+                triangular_first = 1;
+                step_k3 = (g[i3 + 1] - k3) / 2. + k3 - (k1 - k2);
+              }
+              const double next = i3 < n - 1 ? g[i3 + 1] : INFINITY;
+              if (next > k1 + k2 && triangular_last == 0 && i3 > 0) {
+                triangular_last = 1;
+                step_k3 = (k3 - g[i3 - 1]) / 2. + (k1 + k2) - k3;
+              }
+              if (step_k3 == 0) continue;
+              const double scale =
+                  pow(sqrt(1. - pow((k3 * k3 + k1 * k1 - k2 * k2) / 2. / k3 / k1, 2)), m);
+              const double total_factor = 2. * k1 * k2 * k3 * step_k1 * step_k2 * step_k3 *
+                                          (rescale && step_k3 != 0. ? pow(scale, 2) : 1.) *
+                                          2. / PI / 2. / PI / 2. / PI / 2. / PI * spectra_k1 *
+                                          spectra_k2;
+              const double integrand = total_factor * T1[b + i3] * T2[b + i3];
+              result += integrand;
+              result_m += integrand;
+            }
+          }
+        }
+      }
+      cl_m[il * m_size + iM] = result_m;
+    }
+    cl[il] = result;
+  }
   return 0;
 }

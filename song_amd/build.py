@@ -29,6 +29,7 @@ HIP_SOURCES = [
     ("los_serve.hip", []),
     ("los_bessel2_gpu.hip", ["-ffp-contract=off"]),
     ("los_k3int.hip", ["-ffp-contract=off"]),
+    ("los_cl2.hip", ["-ffp-contract=off"]),
 ]
 
 C_FLAGS = ["-O2", "-std=c99", "-D_GNU_SOURCE", "-fPIC", "-fopenmp", "-Wall",

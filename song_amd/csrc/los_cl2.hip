@@ -1,0 +1,251 @@
+// los_cl2.hip — the second-order angular power spectrum integral over the
+// transfer functions on the GPU (gfx950), FP64: the (k1, k2, k3) sum of SONG's
+// spectra2_cls (source/spectra2.c:241-590, the k3 loop :466-540), for every l
+// and m of the run:
+//
+//   C_l^m = sum_{k1} sum_{k2<=k1} sum_{k3} 2 k1 k2 k3 dk1 dk2 dk3 [scale^2]
+//           * 2/pi/2/pi/2/pi/2/pi * P1 P2 * T1_lm(k1,k2,k3) T2_lm(k1,k2,k3)
+//
+// with dk1, dk2 the half trapezoid steps of the k grid (dk2 = 0 for k1 = 0,
+// the k1 = k2 end of the k2 range taking the left half step only), dk3 the
+// half trapezoid steps of the pair's k3 grid with the triangle-edge
+// corrections (:486-500), P = 2 pi^2 / k^3 times the primordial spectrum, and
+// scale^2 = (1 - cos^2)^m, cos = (k3^2 + k1^2 - k2^2) / (2 k3 k1), when the
+// sources were rescaled (ppt2->rescale_cmb_sources).
+//
+// Organisation: one workgroup of 256 threads per (l, m); each thread takes
+// pairs in turn and walks the pair's k3 grid in order (the triangle-edge
+// corrections are sequential state); the per-thread sums are reduced in LDS
+// in a fixed order, so results are deterministic. The sum order differs from
+// the reference's sequential one: results agree to rounding (relative 1e-12).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "los_internal.h"
+
+#define CL2_THREADS 256
+
+#define CCHK(call)                                                                           \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      snprintf(errmsg, _ERRORMSGSIZE_, "%s: %s failed: %s", __func__, #call,                 \
+               hipGetErrorString(e_));                                                       \
+      return _FAILURE_;                                                                      \
+    }                                                                                        \
+  } while (0)
+
+struct los_cl2 {
+  int device;
+  int k_size, n_pairs, n_l, m_size;
+  long n_k3;
+  int rescale;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1;
+  double *k, *pk, *k3, *t1, *t2, *out;
+  long *pair_off;
+  int *pair_k1, *pair_k2, *m, *used;
+};
+
+// Weight of one (k1, k2, k3) term without the transfers; the reference's
+// factor order (:502-511).
+__device__ __forceinline__ double cl2_factor(double k1, double k2, double k3, double s1,
+                                             double s2, double s3, double sp1, double sp2,
+                                             int m, int rescale)
+{
+  double sc2 = 1.;
+  if (rescale) {
+    const double c = (k3 * k3 + k1 * k1 - k2 * k2) / 2. / k3 / k1;
+    const double scale = pow(sqrt(1. - c * c), m);
+    sc2 = scale * scale;
+  }
+  return 2. * k1 * k2 * k3 * s1 * s2 * s3 * sc2 * 2. / M_PI / 2. / M_PI / 2. / M_PI / 2. /
+         M_PI * sp1 * sp2;
+}
+
+__global__ void __launch_bounds__(CL2_THREADS)
+los_cl2_kernel(const double *__restrict__ kk, const double *__restrict__ pk, int k_size,
+               const long *__restrict__ pair_off, const int *__restrict__ pair_k1,
+               const int *__restrict__ pair_k2, int n_pairs, const double *__restrict__ k3g,
+               long n_k3, const double *__restrict__ t1, const double *__restrict__ t2,
+               const int *__restrict__ mv, int m_size, const int *__restrict__ used,
+               int rescale, double *__restrict__ out)
+{
+  __shared__ double part[CL2_THREADS];
+  const int iM = blockIdx.x % m_size;
+  const int il = blockIdx.x / m_size;
+  const int m = mv[iM];
+  const long row = (long)il * m_size + iM;
+  double acc = 0.;
+  if (used[row]) {
+    const double *T1 = t1 + row * n_k3;
+    const double *T2 = t2 + row * n_k3;
+    for (int p = threadIdx.x; p < n_pairs; p += CL2_THREADS) {
+      const int i1 = pair_k1[p], i2 = pair_k2[p];
+      const double k1 = kk[i1], k2 = kk[i2];
+      // half trapezoid steps in k1 and k2 (:375-380, :398-404)
+      double s1, s2;
+      if (i1 == 0) s1 = (kk[1] - kk[0]) / 2.;
+      else if (i1 == k_size - 1) s1 = (kk[k_size - 1] - kk[k_size - 2]) / 2.;
+      else s1 = (kk[i1 + 1] - kk[i1 - 1]) / 2.;
+      if (i1 == 0) s2 = 0.;
+      else if (i2 == 0) s2 = (kk[1] - kk[0]) / 2.;
+      else if (i2 == i1) s2 = (kk[i1] - kk[i1 - 1]) / 2.;
+      else s2 = (kk[i2 + 1] - kk[i2 - 1]) / 2.;
+      const double sp1 = 2 * M_PI * M_PI / (k1 * k1 * k1) * pk[i1];
+      const double sp2 = 2 * M_PI * M_PI / (k2 * k2 * k2) * pk[i2];
+      const long b = pair_off[p];
+      const int n = (int)(pair_off[p + 1] - b);
+      const double *g = k3g + b;
+      int first = 0, last = 0;
+      for (int i = 0; i < n; ++i) {
+        const double k3 = g[i];
+        double s3;
+        if (i == 0) s3 = (g[1] - g[0]) / 2.;
+        else if (i == n - 1) s3 = (g[n - 1] - g[n - 2]) / 2.;
+        else s3 = (g[i + 1] - g[i - 1]) / 2.;
+        if (k3 < k1 - k2 || k3 > k1 + k2) s3 = 0.;
+        if (k3 >= k1 - k2 && first == 0 && i < n - 1) {
+          first = 1;
+          s3 = (g[i + 1] - k3) / 2. + k3 - (k1 - k2);
+        }
+        // the point after the last is taken as beyond k1 + k2
+        const bool next_out = (i == n - 1) || (g[i + 1] > k1 + k2);
+        if (next_out && last == 0 && i > 0) {
+          last = 1;
+          s3 = (k3 - g[i - 1]) / 2. + (k1 + k2) - k3;
+        }
+        if (s3 == 0) continue;
+        const double f = cl2_factor(k1, k2, k3, s1, s2, s3, sp1, sp2, m, rescale);
+        acc += f * T1[b + i] * T2[b + i];
+      }
+    }
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = CL2_THREADS / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[row] = part[0];
+}
+
+template <class T>
+static hipError_t cl2_upload(T **dst, const T *src, long n)
+{
+  hipError_t e = hipMalloc((void **)dst, (size_t)(n > 0 ? n : 1) * sizeof(T));
+  if (e != hipSuccess || n <= 0) return e;
+  return hipMemcpy(*dst, src, (size_t)n * sizeof(T), hipMemcpyHostToDevice);
+}
+
+extern "C" {
+
+void los_cl2_destroy(los_cl2 *h)
+{
+  if (!h) return;
+  hipSetDevice(h->device);
+  void *bufs[] = {h->k, h->pk, h->k3, h->t1, h->t2, h->out, h->pair_off,
+                  h->pair_k1, h->pair_k2, h->m, h->used};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  free(h);
+}
+
+int los_cl2_create(const los_cl2_desc *d, int device, los_cl2 **out, char *errmsg)
+{
+  *out = NULL;
+  los_test(d->k_size < 2, errmsg, "the k grid needs two points");
+  los_test(d->n_l < 0 || d->m_size < 1, errmsg, "bad l or m counts");
+  const int n_pairs = d->k_size * (d->k_size + 1) / 2;
+  los_test(d->pair_off[0] != 0, errmsg, "pair_off[0] must be 0");
+  // source/spectra2.c:463-465
+  for (int p = 0; p < n_pairs; ++p)
+    los_test(d->pair_off[p + 1] - d->pair_off[p] < 2, errmsg,
+             "pair %d: integration grid has less than two elements, cannot use trapezoidal "
+             "integration", p);
+  for (int i = 0; i < d->k_size; ++i)
+    los_test(!(d->k[i] > 0), errmsg, "k[%d] = %g must be positive", i, d->k[i]);
+  const long n_k3 = d->pair_off[n_pairs];
+  for (long i = 0; i < n_k3; ++i)
+    los_test(!(d->k3[i] > 0), errmsg, "k3[%ld] = %g must be positive", i, d->k3[i]);
+  std::vector<int> pk1(n_pairs), pk2(n_pairs);
+  for (int i1 = 0, p = 0; i1 < d->k_size; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2, ++p) {
+      pk1[p] = i1;
+      pk2[p] = i2;
+    }
+  const long rows = (long)d->n_l * d->m_size;
+  los_cl2 *h = (los_cl2 *)calloc(1, sizeof *h);
+  los_test(h == NULL, errmsg, "out of host memory");
+  h->device = device;
+  h->k_size = d->k_size;
+  h->n_pairs = n_pairs;
+  h->n_l = d->n_l;
+  h->m_size = d->m_size;
+  h->n_k3 = n_k3;
+  h->rescale = d->rescale;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreate(&h->stream);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = cl2_upload(&h->k, d->k, (long)d->k_size);
+  if (e == hipSuccess) e = cl2_upload(&h->pk, d->pk, (long)d->k_size);
+  if (e == hipSuccess) e = cl2_upload(&h->pair_off, d->pair_off, (long)n_pairs + 1);
+  if (e == hipSuccess) e = cl2_upload(&h->pair_k1, pk1.data(), (long)n_pairs);
+  if (e == hipSuccess) e = cl2_upload(&h->pair_k2, pk2.data(), (long)n_pairs);
+  if (e == hipSuccess) e = cl2_upload(&h->k3, d->k3, n_k3);
+  if (e == hipSuccess) e = cl2_upload(&h->t1, d->transfer1, rows * n_k3);
+  if (e == hipSuccess) e = cl2_upload(&h->t2, d->transfer2, rows * n_k3);
+  if (e == hipSuccess) e = cl2_upload(&h->m, d->m, (long)d->m_size);
+  if (e == hipSuccess) e = cl2_upload(&h->used, d->used, rows);
+  if (e == hipSuccess) e = hipMalloc((void **)&h->out, (size_t)(rows > 0 ? rows : 1) * 8);
+  if (e != hipSuccess) {
+    snprintf(errmsg, _ERRORMSGSIZE_, "los_cl2_create: %s", hipGetErrorString(e));
+    los_cl2_destroy(h);
+    return _FAILURE_;
+  }
+  *out = h;
+  return _SUCCESS_;
+}
+
+int los_cl2_run(los_cl2 *h, int reps, float *ms, char *errmsg)
+{
+  CCHK(hipSetDevice(h->device));
+  if (ms) *ms = 0;
+  const long rows = (long)h->n_l * h->m_size;
+  if (rows == 0) return _SUCCESS_;
+  if (reps < 1) reps = 1;
+  CCHK(hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < reps; ++i)
+    hipLaunchKernelGGL(los_cl2_kernel, dim3((unsigned)rows), dim3(CL2_THREADS), 0, h->stream,
+                       h->k, h->pk, h->k_size, h->pair_off, h->pair_k1, h->pair_k2, h->n_pairs,
+                       h->k3, h->n_k3, h->t1, h->t2, h->m, h->m_size, h->used, h->rescale,
+                       h->out);
+  CCHK(hipGetLastError());
+  CCHK(hipEventRecord(h->ev1, h->stream));
+  CCHK(hipEventSynchronize(h->ev1));
+  float t = 0;
+  CCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  if (ms) *ms = t / reps;
+  return _SUCCESS_;
+}
+
+int los_cl2_result(los_cl2 *h, double *cl_m, long n, char *errmsg)
+{
+  const long rows = (long)h->n_l * h->m_size;
+  los_test(n != rows, errmsg, "los_cl2_result: %ld values asked, %ld held", n, rows);
+  if (rows == 0) return _SUCCESS_;
+  CCHK(hipSetDevice(h->device));
+  CCHK(hipMemcpy(cl_m, h->out, (size_t)rows * sizeof(double), hipMemcpyDeviceToHost));
+  return _SUCCESS_;
+}
+
+}  // extern "C"

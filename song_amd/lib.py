@@ -68,6 +68,16 @@ class CK3IntDesc(C.Structure):
                 ("r", C.POINTER(C.c_double)), ("coef", C.c_double)]
 
 
+class CCl2Desc(C.Structure):
+    """Mirror of ``los_cl2_desc`` (include/song_los.h)."""
+    _fields_ = [("k_size", C.c_int), ("k", C.POINTER(C.c_double)),
+                ("pk", C.POINTER(C.c_double)), ("pair_off", C.POINTER(C.c_long)),
+                ("k3", C.POINTER(C.c_double)), ("n_l", C.c_int), ("m_size", C.c_int),
+                ("m", C.POINTER(C.c_int)), ("transfer1", C.POINTER(C.c_double)),
+                ("transfer2", C.POINTER(C.c_double)), ("used", C.POINTER(C.c_int)),
+                ("rescale", C.c_int)]
+
+
 def _declare(lib: C.CDLL) -> C.CDLL:
     vp, i, d, ch = C.c_void_p, C.c_int, C.c_double, C.c_char_p
     sig = {
@@ -132,6 +142,10 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_k3int_result": (i, [vp, C.POINTER(d), C.c_long, ch]),
         "los_k3int_counts": (i, [vp, C.POINTER(d), i]),
         "los_k3int_destroy": (None, [vp]),
+        "los_cl2_create": (i, [C.POINTER(CCl2Desc), i, C.POINTER(vp), ch]),
+        "los_cl2_run": (i, [vp, i, C.POINTER(C.c_float), ch]),
+        "los_cl2_result": (i, [vp, C.POINTER(d), C.c_long, ch]),
+        "los_cl2_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -608,6 +622,62 @@ class K3Integral:
     def close(self) -> None:
         if self.ptr:
             self.lib.los_k3int_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Cl2:
+    """Second-order C_l sum over (k1, k2, k3) on the GPU (los_cl2_*).
+
+    k, pk (primordial spectrum at k), pair_off (k1-major pairs, k2 <= k1), k3 (concatenated
+    grids), m (values), transfer1/transfer2 [n_l][m_size][n_k3], used [n_l][m_size], rescale.
+    """
+
+    def __init__(self, k, pk, pair_off, k3, m, transfer1, transfer2, used, rescale=0,
+                 device: int = 0):
+        self.lib = load()
+        self._keep = [np.ascontiguousarray(k, dtype=np.float64),
+                      np.ascontiguousarray(pk, dtype=np.float64),
+                      np.ascontiguousarray(pair_off, dtype=np.int64),
+                      np.ascontiguousarray(k3, dtype=np.float64),
+                      np.ascontiguousarray(m, dtype=np.int32),
+                      np.ascontiguousarray(transfer1, dtype=np.float64),
+                      np.ascontiguousarray(transfer2, dtype=np.float64),
+                      np.ascontiguousarray(used, dtype=np.int32)]
+        kk, pp, po, k3a, mm, t1, t2, uu = self._keep
+        self.n_l, self.m_size = uu.shape
+        dp, ip, lp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+        self.desc = CCl2Desc(kk.size, kk.ctypes.data_as(dp), pp.ctypes.data_as(dp),
+                             po.ctypes.data_as(lp), k3a.ctypes.data_as(dp), self.n_l,
+                             self.m_size, mm.ctypes.data_as(ip), t1.ctypes.data_as(dp),
+                             t2.ctypes.data_as(dp), uu.ctypes.data_as(ip), int(rescale))
+        self.ptr = C.c_void_p()
+        err = _err()
+        check(self.lib.los_cl2_create(C.byref(self.desc), device, C.byref(self.ptr), err),
+              err, "los_cl2_create")
+
+    def run(self, reps: int = 1) -> float:
+        ms = C.c_float()
+        err = _err()
+        check(self.lib.los_cl2_run(self.ptr, reps, C.byref(ms), err), err, "los_cl2_run")
+        return ms.value
+
+    def result(self) -> np.ndarray:
+        """cl_m [n_l][m_size]; C_l is the sum over the last axis."""
+        out = np.empty((self.n_l, self.m_size), dtype=np.float64)
+        err = _err()
+        check(self.lib.los_cl2_result(self.ptr, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                      out.size, err), err, "los_cl2_result")
+        return out
+
+    def close(self) -> None:
+        if self.ptr:
+            self.lib.los_cl2_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
     def __del__(self):
