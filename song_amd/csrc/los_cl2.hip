@@ -13,11 +13,16 @@
 // scale^2 = (1 - cos^2)^m, cos = (k3^2 + k1^2 - k2^2) / (2 k3 k1), when the
 // sources were rescaled (ppt2->rescale_cmb_sources).
 //
-// Organisation: one workgroup of 256 threads per (l, m); each thread takes
-// pairs in turn and walks the pair's k3 grid in order (the triangle-edge
-// corrections are sequential state); the per-thread sums are reduced in LDS
-// in a fixed order, so results are deterministic. The sum order differs from
-// the reference's sequential one: results agree to rounding (relative 1e-12).
+// Organisation: the per-pair quantities (k1, k2, their half steps and P
+// values, computed on the host with the reference's expressions) and each
+// pair's two triangle-edge points (the first point with k3 >= k1 - k2 before
+// the last, and the first point after the first whose successor lies beyond
+// k1 + k2, the point past the end counting as beyond) are resolved once, so
+// every k3 term is independent: los_cl2_partial runs CL2_CHUNKS workgroups of
+// 256 threads per (l, m) row over the flat k3 index and reduces in LDS in a
+// fixed order; los_cl2_finish adds a row's chunk sums in order. Results are
+// deterministic; the sum order differs from the reference's sequential one,
+// so they agree to rounding (relative 1e-12 of the sum of |terms|).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -40,6 +45,8 @@
     }                                                                                        \
   } while (0)
 
+#define CL2_CHUNKS 32
+
 struct los_cl2 {
   int device;
   int k_size, n_pairs, n_l, m_size;
@@ -47,83 +54,69 @@ struct los_cl2 {
   int rescale;
   hipStream_t stream;
   hipEvent_t ev0, ev1;
-  double *k, *pk, *k3, *t1, *t2, *out;
+  double *k3, *t1, *t2, *out, *partial, *pq;  // pq: per pair k1 k2 s1 s2 sp1 sp2
   long *pair_off;
-  int *pair_k1, *pair_k2, *m, *used;
+  int *pair_of, *edge, *m, *used;             // edge: per pair first, last
 };
 
-// Weight of one (k1, k2, k3) term without the transfers; the reference's
-// factor order (:502-511).
+// One (k1, k2, k3) term without the transfers; the reference's factor order
+// (:502-511). sqrt of a negative 1 - cos^2 (rounding at a triangle edge) is
+// NaN there, and pow(NaN, 0) = 1, as in the reference.
 __device__ __forceinline__ double cl2_factor(double k1, double k2, double k3, double s1,
                                              double s2, double s3, double sp1, double sp2,
                                              int m, int rescale)
 {
   double sc2 = 1.;
-  if (rescale) {
+  if (rescale && m != 0) {
     const double c = (k3 * k3 + k1 * k1 - k2 * k2) / 2. / k3 / k1;
-    const double scale = pow(sqrt(1. - c * c), m);
-    sc2 = scale * scale;
+    const double w = 1. - c * c;
+    if (w < 0) {
+      sc2 = __builtin_nan("");
+    } else {
+      sc2 = w;
+      for (int i = 1; i < m; ++i) sc2 *= w;
+    }
   }
   return 2. * k1 * k2 * k3 * s1 * s2 * s3 * sc2 * 2. / M_PI / 2. / M_PI / 2. / M_PI / 2. /
          M_PI * sp1 * sp2;
 }
 
 __global__ void __launch_bounds__(CL2_THREADS)
-los_cl2_kernel(const double *__restrict__ kk, const double *__restrict__ pk, int k_size,
-               const long *__restrict__ pair_off, const int *__restrict__ pair_k1,
-               const int *__restrict__ pair_k2, int n_pairs, const double *__restrict__ k3g,
-               long n_k3, const double *__restrict__ t1, const double *__restrict__ t2,
-               const int *__restrict__ mv, int m_size, const int *__restrict__ used,
-               int rescale, double *__restrict__ out)
+los_cl2_partial(const long *__restrict__ pair_off, const int *__restrict__ pair_of,
+                const int *__restrict__ edge, const double *__restrict__ pq,
+                const double *__restrict__ k3g, long n_k3, const double *__restrict__ t1,
+                const double *__restrict__ t2, const int *__restrict__ mv, int m_size,
+                const int *__restrict__ used, int rescale, double *__restrict__ partial)
 {
   __shared__ double part[CL2_THREADS];
-  const int iM = blockIdx.x % m_size;
-  const int il = blockIdx.x / m_size;
-  const int m = mv[iM];
-  const long row = (long)il * m_size + iM;
+  const int chunk = blockIdx.x % CL2_CHUNKS;
+  const long row = blockIdx.x / CL2_CHUNKS;
+  const int m = mv[row % m_size];
   double acc = 0.;
   if (used[row]) {
     const double *T1 = t1 + row * n_k3;
     const double *T2 = t2 + row * n_k3;
-    for (int p = threadIdx.x; p < n_pairs; p += CL2_THREADS) {
-      const int i1 = pair_k1[p], i2 = pair_k2[p];
-      const double k1 = kk[i1], k2 = kk[i2];
-      // half trapezoid steps in k1 and k2 (:375-380, :398-404)
-      double s1, s2;
-      if (i1 == 0) s1 = (kk[1] - kk[0]) / 2.;
-      else if (i1 == k_size - 1) s1 = (kk[k_size - 1] - kk[k_size - 2]) / 2.;
-      else s1 = (kk[i1 + 1] - kk[i1 - 1]) / 2.;
-      if (i1 == 0) s2 = 0.;
-      else if (i2 == 0) s2 = (kk[1] - kk[0]) / 2.;
-      else if (i2 == i1) s2 = (kk[i1] - kk[i1 - 1]) / 2.;
-      else s2 = (kk[i2 + 1] - kk[i2 - 1]) / 2.;
-      const double sp1 = 2 * M_PI * M_PI / (k1 * k1 * k1) * pk[i1];
-      const double sp2 = 2 * M_PI * M_PI / (k2 * k2 * k2) * pk[i2];
+    for (long i = (long)chunk * CL2_THREADS + threadIdx.x; i < n_k3;
+         i += (long)CL2_CHUNKS * CL2_THREADS) {
+      const int p = pair_of[i];
       const long b = pair_off[p];
       const int n = (int)(pair_off[p + 1] - b);
+      const int j = (int)(i - b);
       const double *g = k3g + b;
-      int first = 0, last = 0;
-      for (int i = 0; i < n; ++i) {
-        const double k3 = g[i];
-        double s3;
-        if (i == 0) s3 = (g[1] - g[0]) / 2.;
-        else if (i == n - 1) s3 = (g[n - 1] - g[n - 2]) / 2.;
-        else s3 = (g[i + 1] - g[i - 1]) / 2.;
-        if (k3 < k1 - k2 || k3 > k1 + k2) s3 = 0.;
-        if (k3 >= k1 - k2 && first == 0 && i < n - 1) {
-          first = 1;
-          s3 = (g[i + 1] - k3) / 2. + k3 - (k1 - k2);
-        }
-        // the point after the last is taken as beyond k1 + k2
-        const bool next_out = (i == n - 1) || (g[i + 1] > k1 + k2);
-        if (next_out && last == 0 && i > 0) {
-          last = 1;
-          s3 = (k3 - g[i - 1]) / 2. + (k1 + k2) - k3;
-        }
-        if (s3 == 0) continue;
-        const double f = cl2_factor(k1, k2, k3, s1, s2, s3, sp1, sp2, m, rescale);
-        acc += f * T1[b + i] * T2[b + i];
-      }
+      const double *q = pq + 6 * (long)p;
+      const double k1 = q[0], k2 = q[1];
+      const double k3 = g[j];
+      double s3;
+      // half trapezoid step and triangle-edge corrections (:486-500)
+      if (j == edge[2 * p + 1]) s3 = (k3 - g[j - 1]) / 2. + (k1 + k2) - k3;
+      else if (j == edge[2 * p]) s3 = (g[j + 1] - k3) / 2. + k3 - (k1 - k2);
+      else if (k3 < k1 - k2 || k3 > k1 + k2) s3 = 0.;
+      else if (j == 0) s3 = (g[1] - g[0]) / 2.;
+      else if (j == n - 1) s3 = (g[n - 1] - g[n - 2]) / 2.;
+      else s3 = (g[j + 1] - g[j - 1]) / 2.;
+      if (s3 == 0) continue;
+      const double f = cl2_factor(k1, k2, k3, q[2], q[3], s3, q[4], q[5], m, rescale);
+      acc += f * T1[i] * T2[i];
     }
   }
   part[threadIdx.x] = acc;
@@ -132,7 +125,17 @@ los_cl2_kernel(const double *__restrict__ kk, const double *__restrict__ pk, int
     if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[row] = part[0];
+  if (threadIdx.x == 0) partial[row * CL2_CHUNKS + chunk] = part[0];
+}
+
+__global__ void __launch_bounds__(64)
+los_cl2_finish(const double *__restrict__ partial, long rows, double *__restrict__ out)
+{
+  const long row = (long)blockIdx.x * 64 + threadIdx.x;
+  if (row >= rows) return;
+  double v = 0.;
+  for (int c = 0; c < CL2_CHUNKS; ++c) v += partial[row * CL2_CHUNKS + c];
+  out[row] = v;
 }
 
 template <class T>
@@ -149,8 +152,8 @@ void los_cl2_destroy(los_cl2 *h)
 {
   if (!h) return;
   hipSetDevice(h->device);
-  void *bufs[] = {h->k, h->pk, h->k3, h->t1, h->t2, h->out, h->pair_off,
-                  h->pair_k1, h->pair_k2, h->m, h->used};
+  void *bufs[] = {h->k3, h->t1, h->t2, h->out, h->partial, h->pq,
+                  h->pair_off, h->pair_of, h->edge, h->m, h->used};
   for (void *b : bufs)
     if (b) hipFree(b);
   if (h->ev0) hipEventDestroy(h->ev0);
@@ -176,11 +179,43 @@ int los_cl2_create(const los_cl2_desc *d, int device, los_cl2 **out, char *errms
   const long n_k3 = d->pair_off[n_pairs];
   for (long i = 0; i < n_k3; ++i)
     los_test(!(d->k3[i] > 0), errmsg, "k3[%ld] = %g must be positive", i, d->k3[i]);
-  std::vector<int> pk1(n_pairs), pk2(n_pairs);
-  for (int i1 = 0, p = 0; i1 < d->k_size; ++i1)
+  // per pair: k1, k2, half steps and P values (:375-380, :388, :398-404, :420),
+  // and the two triangle-edge points of its k3 grid (:490-500)
+  const double PI = M_PI;
+  const double *k = d->k;
+  const int ks = d->k_size;
+  std::vector<double> pq(6 * (size_t)n_pairs);
+  std::vector<int> edge(2 * (size_t)n_pairs), pair_of(n_k3 > 0 ? n_k3 : 1);
+  for (int i1 = 0, p = 0; i1 < ks; ++i1)
     for (int i2 = 0; i2 <= i1; ++i2, ++p) {
-      pk1[p] = i1;
-      pk2[p] = i2;
+      double s1, s2;
+      if (i1 == 0) s1 = (k[1] - k[0]) / 2.;
+      else if (i1 == ks - 1) s1 = (k[ks - 1] - k[ks - 2]) / 2.;
+      else s1 = (k[i1 + 1] - k[i1 - 1]) / 2.;
+      if (i1 == 0) s2 = 0.;
+      else if (i2 == 0) s2 = (k[1] - k[0]) / 2.;
+      else if (i2 == i1) s2 = (k[i1] - k[i1 - 1]) / 2.;
+      else s2 = (k[i2 + 1] - k[i2 - 1]) / 2.;
+      const double k1 = k[i1], k2 = k[i2];
+      double *q = &pq[6 * (size_t)p];
+      q[0] = k1;
+      q[1] = k2;
+      q[2] = s1;
+      q[3] = s2;
+      q[4] = 2 * PI * PI / (k1 * k1 * k1) * d->pk[i1];
+      q[5] = 2 * PI * PI / (k2 * k2 * k2) * d->pk[i2];
+      const long b = d->pair_off[p];
+      const int n = (int)(d->pair_off[p + 1] - b);
+      const double *g = d->k3 + b;
+      int first = -1, last = -1;
+      for (int j = 0; j < n; ++j) {
+        if (first < 0 && g[j] >= k1 - k2 && j < n - 1) first = j;
+        const bool next_out = (j == n - 1) || (g[j + 1] > k1 + k2);
+        if (last < 0 && next_out && j > 0) last = j;
+        pair_of[b + j] = p;
+      }
+      edge[2 * (size_t)p] = first;
+      edge[2 * (size_t)p + 1] = last;
     }
   const long rows = (long)d->n_l * d->m_size;
   los_cl2 *h = (los_cl2 *)calloc(1, sizeof *h);
@@ -196,17 +231,18 @@ int los_cl2_create(const los_cl2_desc *d, int device, los_cl2 **out, char *errms
   if (e == hipSuccess) e = hipStreamCreate(&h->stream);
   if (e == hipSuccess) e = hipEventCreate(&h->ev0);
   if (e == hipSuccess) e = hipEventCreate(&h->ev1);
-  if (e == hipSuccess) e = cl2_upload(&h->k, d->k, (long)d->k_size);
-  if (e == hipSuccess) e = cl2_upload(&h->pk, d->pk, (long)d->k_size);
+  if (e == hipSuccess) e = cl2_upload(&h->pq, pq.data(), 6L * n_pairs);
+  if (e == hipSuccess) e = cl2_upload(&h->edge, edge.data(), 2L * n_pairs);
+  if (e == hipSuccess) e = cl2_upload(&h->pair_of, pair_of.data(), n_k3);
   if (e == hipSuccess) e = cl2_upload(&h->pair_off, d->pair_off, (long)n_pairs + 1);
-  if (e == hipSuccess) e = cl2_upload(&h->pair_k1, pk1.data(), (long)n_pairs);
-  if (e == hipSuccess) e = cl2_upload(&h->pair_k2, pk2.data(), (long)n_pairs);
   if (e == hipSuccess) e = cl2_upload(&h->k3, d->k3, n_k3);
   if (e == hipSuccess) e = cl2_upload(&h->t1, d->transfer1, rows * n_k3);
   if (e == hipSuccess) e = cl2_upload(&h->t2, d->transfer2, rows * n_k3);
   if (e == hipSuccess) e = cl2_upload(&h->m, d->m, (long)d->m_size);
   if (e == hipSuccess) e = cl2_upload(&h->used, d->used, rows);
   if (e == hipSuccess) e = hipMalloc((void **)&h->out, (size_t)(rows > 0 ? rows : 1) * 8);
+  if (e == hipSuccess)
+    e = hipMalloc((void **)&h->partial, (size_t)(rows > 0 ? rows : 1) * CL2_CHUNKS * 8);
   if (e != hipSuccess) {
     snprintf(errmsg, _ERRORMSGSIZE_, "los_cl2_create: %s", hipGetErrorString(e));
     los_cl2_destroy(h);
@@ -224,11 +260,14 @@ int los_cl2_run(los_cl2 *h, int reps, float *ms, char *errmsg)
   if (rows == 0) return _SUCCESS_;
   if (reps < 1) reps = 1;
   CCHK(hipEventRecord(h->ev0, h->stream));
-  for (int i = 0; i < reps; ++i)
-    hipLaunchKernelGGL(los_cl2_kernel, dim3((unsigned)rows), dim3(CL2_THREADS), 0, h->stream,
-                       h->k, h->pk, h->k_size, h->pair_off, h->pair_k1, h->pair_k2, h->n_pairs,
-                       h->k3, h->n_k3, h->t1, h->t2, h->m, h->m_size, h->used, h->rescale,
-                       h->out);
+  los_test(rows * CL2_CHUNKS > 0x7fffffffL, errmsg, "los_cl2_run: too many rows");
+  for (int i = 0; i < reps; ++i) {
+    hipLaunchKernelGGL(los_cl2_partial, dim3((unsigned)(rows * CL2_CHUNKS)), dim3(CL2_THREADS),
+                       0, h->stream, h->pair_off, h->pair_of, h->edge, h->pq, h->k3, h->n_k3,
+                       h->t1, h->t2, h->m, h->m_size, h->used, h->rescale, h->partial);
+    hipLaunchKernelGGL(los_cl2_finish, dim3((unsigned)((rows + 63) / 64)), dim3(64), 0,
+                       h->stream, h->partial, rows, h->out);
+  }
   CCHK(hipGetLastError());
   CCHK(hipEventRecord(h->ev1, h->stream));
   CCHK(hipEventSynchronize(h->ev1));
