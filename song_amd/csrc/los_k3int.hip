@@ -25,6 +25,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "los_internal.h"
@@ -174,7 +175,11 @@ int los_k3int_create(const los_k3int_desc *d, int device, los_k3int **out, char 
     dk[e - 1] = d->k3[e - 1] - d->k3[e - 2];
   }
   // algorithmic counts: live terms, bytes (inputs once, used j windows once, outputs)
+  // live terms: T != 0 and k3 r >= x_min; fl(k3 r) is monotone in r, so the
+  // count over r is a binary search on the sorted r grid with the same test
   long live = 0;
+  std::vector<double> rs(d->r, d->r + d->n_r);
+  std::sort(rs.begin(), rs.end());
   std::vector<char> used(d->n_j > 0 ? d->n_j : 1, 0);
   for (int q = 0; q < d->n_q; ++q) {
     const int iL = d->index_L[q];
@@ -184,7 +189,13 @@ int los_k3int_create(const los_k3int_desc *d, int device, los_k3int **out, char 
     const double *T = d->transfer + (long)q * n_k3;
     for (long i = 0; i < n_k3; ++i) {
       if (T[i] == 0.) continue;
-      for (int ir = 0; ir < d->n_r; ++ir) live += (d->k3[i] * d->r[ir] >= xmin);
+      const double k = d->k3[i];
+      int lo = 0, hi = d->n_r;  // first sorted r with k r >= xmin
+      while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (k * rs[mid] >= xmin) hi = mid; else lo = mid + 1;
+      }
+      live += d->n_r - lo;
     }
   }
   double jbytes = 0;
