@@ -129,12 +129,17 @@ def ragged_case(seed, n_pairs=29, n_r=150, abs_M3=1, coef=1.3):
     pair_off = np.concatenate([[0], np.cumsum(counts)])
     k3 = np.concatenate([np.sort(rng.uniform(1e-4, 0.2, c)) for c in counts])
     ls = [0, 2, 5, 40, 700]
-    rows = [0, 1, -1, 2, 3, 4, 1]
+    rows = [0, 1, -1, 2, 3, 4, 1, 5]  # table 5: j_l1 negligible over the whole x grid
     T = rng.standard_normal((len(rows), k3.size))
     T[rng.random(T.shape) < 0.1] = 0.0
     r = np.sort(rng.uniform(0.0, 14000.0, n_r))
     step = 0.3
     j_off, j, x_min = j_tables(ls, 0.2 * 14000.0 + 2.0, step)
+    # source/bessel2.c:1756-1759, :1789-1792: a table whose values are all below the cut holds one zero
+    # at x_min_l1 = x_max; every term of its rows is skipped
+    j_off = np.append(j_off, j_off[-1] + 1)
+    j = np.append(j, 0.0)
+    x_min = np.append(x_min, 15000.0)
     return dict(pair_off=pair_off, k3=k3, transfer=T, index_L=np.array(rows), j_off=j_off,
                 j=j, x_min=x_min, step=step, r=r, abs_M3=abs_M3, coef=coef)
 
@@ -155,6 +160,7 @@ def test_k3int_bit_exact(seed, abs_M3, coef, n_r):
     h.close()
     assert np.count_nonzero(ref) > 0.5 * ref.size
     assert not np.any(got[2]) and not np.any(ref[2])  # skipped row
+    assert not np.any(got[7]) and not np.any(ref[7])  # negligible table
     diff = np.flatnonzero(got != ref)
     assert diff.size == 0, f"{diff.size} of {ref.size} differ; first {diff[:5]}"
 
