@@ -167,6 +167,8 @@ int los_cl2_create(const los_cl2_desc *d, int device, los_cl2 **out, char *errms
   *out = NULL;
   los_test(d->k_size < 2, errmsg, "the k grid needs two points");
   los_test(d->n_l < 0 || d->m_size < 1, errmsg, "bad l or m counts");
+  for (int i = 0; i < d->m_size; ++i)
+    los_test(d->m[i] < 0, errmsg, "m[%d] = %d: azimuthal numbers are non-negative", i, d->m[i]);
   const int n_pairs = d->k_size * (d->k_size + 1) / 2;
   los_test(d->pair_off[0] != 0, errmsg, "pair_off[0] must be 0");
   // source/spectra2.c:463-465

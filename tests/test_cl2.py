@@ -122,6 +122,11 @@ def test_create_rejects_bad_inputs_before_gpu_work():
     t = np.ones((1, 1, 5))
     with pytest.raises(L.LosError, match="less than two"):
         L.Cl2(k, [1.0, 1.0], po, k3, [0], t, t, [[1]])
+    po_ok = np.array([0, 2, 4, 6])
+    k3_ok = np.array([0.001, 0.015, 0.01, 0.02, 0.01, 0.03])
+    t6 = np.ones((1, 1, 6))
+    with pytest.raises(L.LosError, match="non-negative"):
+        L.Cl2(k, [1.0, 1.0], po_ok, k3_ok, [-1], t6, t6, [[1]])
 
 
 def random_case(seed, k_size=10, m=(0, 1, 2), n_l=5):
