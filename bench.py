@@ -319,6 +319,14 @@ def _pair_k3_grids(lib, scen, cfg):
     return sizes, pair_off, k3
 
 
+def _copy_probe(args):
+    """A plain device copy of --copy-probe-mib MiB (HBM counter calibration), if asked."""
+    if args.copy_probe_mib > 0:
+        eng = L.Engine(0)
+        eng.copy_probe(args.copy_probe_mib * 2.0 ** 20)
+        eng.close()
+
+
 def cl2_bench(args, cfg):
     """The second-order C_l sum (spectra2.c:241-590) over the c2 transfers."""
     lib = L.load()
@@ -339,6 +347,7 @@ def cl2_bench(args, cfg):
     used = np.ones((n_l, len(m)), dtype=np.int32)
     print(f"# cl2 inputs ({n_l} l x {len(m)} m, {len(sizes)} pairs, {n_k3} k3) in "
           f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    _copy_probe(args)
     res, ms_tot = {}, 0.0
     for name, t in (("TT", T), ("EE", E)):
         h = L.Cl2(k, pk, pair_off, k3, m, t, t, used, 1)
@@ -361,7 +370,9 @@ def cl2_bench(args, cfg):
         "config": {"workload": f"cl2: {DESCR['cl2']}", "terms": terms,
                    "ms_TT": res["TT"][1], "ms_EE": res["EE"][1]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "los_cl2_partial + los_cl2_finish",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("cl2", "los_cl2_partial + los_cl2_finish"),
+                     "kernel": "los_cl2_partial + los_cl2_finish",
                      "kernel_ms": ms_tot / 2, "alg_bytes": alg_bytes / 2},
     }
     if not args.no_cpu:
@@ -446,6 +457,7 @@ def k3int_bench(args, cfg):
     j = np.concatenate(vals)
     print(f"# k3int inputs ({n_l} rows, {len(sizes)} pairs, {n_k3} k3, {r.size} r) in "
           f"{time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    _copy_probe(args)
     h = L.K3Integral(pair_off, k3, T, np.arange(n_l), j_off, j, x_min, step, r, 1.0)
     for _ in range(max(args.warmup, 1)):
         h.run(1)
@@ -464,7 +476,8 @@ def k3int_bench(args, cfg):
         "config": {"workload": f"k3int: {DESCR['k3int']}", "values": n_val,
                    "live_terms": cnt["live_terms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k3int", "los_k3int_kernel"),
                      "kernel": "los_k3int_kernel", "kernel_ms": ms,
                      "alg_bytes": cnt["alg_bytes"],
                      "valu": {"achieved": flop / (ms * 1e-3) / 1e12, "peak": 78.6,
