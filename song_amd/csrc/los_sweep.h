@@ -188,18 +188,12 @@ los_sweep_prep(los_params P, int ntb, int f_begin, int f_end)
   const int nval = nlive * nmc * NL;
   R *st = stage + (tt * nmc + mc) * NL;
 
+  // the bracket's two columns stay in registers: they are only ever indexed by
+  // the unrolled L (an array whose address is passed on would live in scratch)
   double s0[NL], s1[NL], d0[NL], d1[NL];
   int cur = -2, curp = -1;
   long tstep = 0;
   const double *scol = nullptr, *dcol = nullptr, *kc = nullptr;
-  auto load = [&](int k, double *sv, double *dv) {
-#pragma unroll
-    for (int L = 0; L < NL; ++L) {
-      const long o = (long)max(tp[L], 0) * tstep + (long)k * nt;
-      sv[L] = scol[o];
-      if (cubic) dv[L] = dcol[o];
-    }
-  };
   // the next fine k3's pair, bracket and interpolation point, read one step ahead
   int np = P.fpair[f0], nbr = P.fbr[f0];
   double nks = P.kf[P.fsrc[f0]];
@@ -227,10 +221,19 @@ los_sweep_prep(los_params P, int ntb, int f_begin, int f_end)
           s0[L] = s1[L];
           d0[L] = d1[L];
         }
-        load(br + 1, s1, d1);
       } else {
-        load(br, s0, d0);
-        load(br + 1, s1, d1);
+#pragma unroll
+        for (int L = 0; L < NL; ++L) {
+          const long o = (long)max(tp[L], 0) * tstep + (long)br * nt;
+          s0[L] = scol[o];
+          d0[L] = cubic ? dcol[o] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int L = 0; L < NL; ++L) {
+        const long o = (long)max(tp[L], 0) * tstep + (long)(br + 1) * nt;
+        s1[L] = scol[o];
+        d1[L] = cubic ? dcol[o] : 0.0;
       }
       cur = br;
     }
