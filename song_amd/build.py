@@ -47,17 +47,31 @@ def _run(cmd, cwd=None):
     return r
 
 
+def _jobs() -> int:
+    try:
+        n = int(os.environ.get("MAX_JOBS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(n or (os.cpu_count() or 1), 16))
+
+
 def build_library(out=os.path.join(HERE, "libsong_los.so")):
+    from concurrent.futures import ThreadPoolExecutor
+
     with tempfile.TemporaryDirectory(prefix="songlos_") as tmp:
-        objs = []
+        cmds, objs = [], []
         for src in C_SOURCES:
             obj = os.path.join(tmp, src + ".o")
-            _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+            cmds.append(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
             objs.append(obj)
         for src, extra in HIP_SOURCES:
             obj = os.path.join(tmp, src + ".o")
-            _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+            cmds.append([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
             objs.append(obj)
+        # the translation units are independent: compile them side by side
+        with ThreadPoolExecutor(max_workers=_jobs()) as pool:
+            for _ in pool.map(_run, cmds):
+                pass
         tmp_out = out + ".tmp"
         _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp_out, *objs, "-lgomp", "-lm"])
         os.replace(tmp_out, out)

@@ -12,6 +12,7 @@
 #include "los_internal.h"
 
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 /*
@@ -100,8 +101,8 @@ int los_wigner3j_j1(int j2, int j3, int m2, int m3, int *j1min_out, int *j1max_o
                    ((double)(j) * (j) - dm1 * dm1))
 #define B_(j) (-(2.0 * (j) + 1.0) * (dj2 * (dj2 + 1.0) * dm1 - dj3 * (dj3 + 1.0) * dm1 - \
                                      (double)(j) * ((j) + 1.0) * (dm3 - dm2)))
-  /* Downward pass over the whole range. */
-  double down[64];
+  /* Downward pass over the whole range, in the caller's buffer (n <= out_cap). */
+  double *down = out;
   down[n - 1] = 1.0;
   {
     int j = j1max;
@@ -123,11 +124,13 @@ int los_wigner3j_j1(int j2, int j3, int m2, int m3, int *j1min_out, int *j1max_o
   }
   /* Upward pass, only when the first step is well defined (j1min > 0). */
   if (j1min > 0 && n >= 3) {
-    double up[64];
+    const int mid = n / 2;
+    double up_small[64];
+    double *up = mid + 1 <= 64 ? up_small : (double *)malloc((size_t)(mid + 1) * sizeof(double));
+    if (up == NULL) return -1;
     up[0] = 1.0;
     double fjm1 = 0.0, fj = 1.0;
     int j = j1min;
-    int mid = n / 2;
     for (int idx = 0; idx < mid; ++idx, ++j) {
       double fjp1 = -(B_(j) * fj + (j + 1.0) * A_(j) * fjm1) / ((double)j * A_(j + 1));
       up[idx + 1] = fjp1;
@@ -143,6 +146,7 @@ int los_wigner3j_j1(int j2, int j3, int m2, int m3, int *j1min_out, int *j1max_o
       double r = down[best] / up[best];
       for (int q = 0; q < best; ++q) down[q] = up[q] * r;
     }
+    if (up != up_small) free(up);
   }
 #undef A_
 #undef B_
@@ -151,7 +155,7 @@ int los_wigner3j_j1(int j2, int j3, int m2, int m3, int *j1min_out, int *j1max_o
   double sgn = ((j2 - j3 - m1) % 2 == 0) ? 1.0 : -1.0;
   double scale = sgn / sqrt(norm);
   if (down[n - 1] < 0) scale = -scale;
-  for (int q = 0; q < n; ++q) out[q] = down[q] * scale;
+  for (int q = 0; q < n; ++q) out[q] *= scale;
   return n;
 }
 

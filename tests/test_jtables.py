@@ -43,13 +43,14 @@ def test_spherical_bessel_matches_scipy(x):
 
 
 @pytest.mark.parametrize("j2,j3,m2,m3", [(3, 5, 0, 0), (6, 4, -2, 2), (10, 7, 1, -1),
-                                         (2, 2, 0, 0), (6, 30, 2, -2)])
+                                         (2, 2, 0, 0), (6, 30, 2, -2), (40, 50, 3, -1),
+                                         (70, 45, 0, 0)])
 def test_wigner3j_matches_sympy(j2, j3, m2, m3):
     from sympy.physics.wigner import wigner_3j
     lib = L.load()
-    out = (C.c_double * 128)()
+    out = (C.c_double * 160)()
     a, b = C.c_int(), C.c_int()
-    n = lib.los_wigner3j_j1(j2, j3, m2, m3, C.byref(a), C.byref(b), out, 128)
+    n = lib.los_wigner3j_j1(j2, j3, m2, m3, C.byref(a), C.byref(b), out, 160)
     assert n == b.value - a.value + 1
     m1 = -m2 - m3
     for q in range(n):
@@ -105,3 +106,11 @@ def test_index_xmin_brackets_the_cut():
                     checked += 1
     assert checked > 20
     s.close()
+
+
+def test_wigner3j_rejects_short_buffer():
+    """A j1 range longer than the caller's buffer is refused, not written past its end."""
+    lib = L.load()
+    out = (C.c_double * 8)()
+    a, b = C.c_int(), C.c_int()
+    assert lib.los_wigner3j_j1(10, 12, 0, 0, C.byref(a), C.byref(b), out, 8) == -1
