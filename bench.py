@@ -50,6 +50,12 @@ _C2 = dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=6, l_max_los_p=6, k_s
            k_min=1e-5, k_max=0.1, k3_size=60, tau_size=600, tau_min=200.0, tau_max=14000.0,
            tau0=14200.0, l_size=96, l_max=2000, dk_fine=1.5e-4, xx_max=15000.0, xx_step=0.3)
 WORKLOADS = {
+    # the CPU reference configuration: T only, m = 0, L = 0..4, 16x16 log (k1,k2) in
+    # [1e-4, 0.05], 400 tau, 48 l in [2, 1000], fine k3 step 2e-4
+    "c1": dict(has_T=1, has_E=0, m_list=[0], l_max_los_t=4, l_max_los_p=4, k_size=16,
+               k_min=1e-4, k_max=0.05, k3_size=60, tau_size=400, tau_min=200.0,
+               tau_max=14000.0, tau0=14200.0, l_size=48, l_max=1000, dk_fine=2e-4,
+               xx_max=15000.0, xx_step=0.3),
     "c2": _C2,
     "c3": dict(_C2, k_size=64, k_log=2, k_max=0.2, k3_size=8, dk_fine=3.5e-4, xx_max=6000.0),
     "c4": dict(_C2, l_size=128, l_max=3000),
@@ -66,6 +72,8 @@ PAIRS = {"c3": 2048}  # workloads that run an explicit list of pairs
 GRAPH = {"c3"}        # workloads whose steps replay a captured graph
 
 DESCR = {
+    "c1": ("plain FP64 LOS, T only m=0 L=0..4, 16x16 log (k1,k2) in [1e-4,0.05], 60 coarse "
+           "k3/pair, dk=2e-4, 400 tau, 48 l in [2,1000]"),
     "c2": ("plain FP64 LOS, T+E+B m=0,1,2 L=0..6, 32x32 log (k1,k2) in [1e-5,0.1], 60 coarse "
            "k3/pair, dk=1.5e-4, 600 tau, 96 l in [2,2000]"),
     "c3": ("batched: 2048 pairs of 64 log-uniform k in [1e-5,0.2], fine k3 8..400 per pair, "
@@ -123,44 +131,118 @@ def select_pairs(cfg: L.Config, n: int):
     return [allp[i] for i in pick]
 
 
-def cpu_baseline(cfg: L.Config, pairs, target_s: float):
-    """Time the CPU oracle on a bounded sample of the workload's pairs."""
-    opath = os.path.join(ROOT, "oracle", "liblos_oracle.so")
-    o = C.CDLL(opath)
+def cpu_info():
+    """CPU model and CPU counts of this host (lscpu), and the CPUs this process may use."""
+    info = {"model": None, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0))}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def _oracle_lib(debug: bool):
+    name = "liblos_oracle.so" if debug else "liblos_oracle_nodebug.so"
+    o = C.CDLL(os.path.join(ROOT, "oracle", name))
     o.los_ref_run_pairs.argtypes = [C.c_void_p] * 8 + [C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                                       C.c_int, C.POINTER(C.c_long)]
+                                                       C.c_int, C.POINTER(C.c_long),
+                                                       C.POINTER(C.c_double)]
     o.los_ref_run_pairs.restype = C.c_int
     o.los_ref_sizeof_transfers2.restype = C.c_long
     o.los_ref_transfer2_free.argtypes = [C.c_void_p] * 3
+    return o
+
+
+def _oracle_pass(o, scen, chunk, out=None):
+    """One pass of the oracle over the pairs `chunk` (its transfer2_init loop body per pair:
+    k3 list, k3 interpolation, OpenMP static loop over k3); (elements, seconds)."""
+    st = scen.s
+    size = o.los_ref_sizeof_transfers2()
+    tr2 = C.create_string_buffer(size)
+    # the stage settings the oracle reads come from the scenario's transfers2
+    C.memmove(tr2, st["ptr2"].value, size)
+    k1 = (C.c_int * len(chunk))(*[p[0] for p in chunk])
+    k2 = (C.c_int * len(chunk))(*[p[1] for p in chunk])
+    n = C.c_long(0)
+    optr = out.ctypes.data_as(C.POINTER(C.c_double)) if out is not None else None
+    t0 = time.perf_counter()
+    rc = o.los_ref_run_pairs(st["ppr"], st["ppr2"], st["pba"], st["pth"], st["ppt2"], st["pbs"],
+                             st["pbs2"], C.cast(tr2, C.c_void_p), k1, k2, len(chunk), C.byref(n),
+                             optr)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError("oracle sample failed")
+    o.los_ref_transfer2_free(st["ppr2"], st["ppt2"], C.cast(tr2, C.c_void_p))
+    return n.value, dt
+
+
+def cpu_baseline(cfg: L.Config, pairs, target_s: float, tt2: int, sizes_of):
+    """The CPU oracle timed on a bounded sample of the workload's pairs.
+
+    Protocol: OpenMP on every CPU this process may use (OMP_PROC_BIND=close,
+    OMP_PLACES=cores, set before any OpenMP runtime starts), the sample grown in pair
+    order (spread over the pair list) until one pass takes about target_s / 12, then for
+    each build (-DDEBUG, the reference's default, and without) one warm-up pass and the
+    median of 5 timed passes. Returns the baseline record and the sample's oracle rows
+    {pair: [tt][k3]} (from the DEBUG build) for the parity check."""
     scen = L.Scenario(cfg)
-    s = scen.s
-    # spread the sample over the whole pair list; grow it until target_s is used
-    total_el, total_t, done, stride = 0, 0.0, 0, 24
+    stride = 24
     order = pairs[::stride] + [p for q, p in enumerate(pairs) if q % stride]
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    batch = 4
-    while done < len(order) and total_t < target_s:
-        chunk = order[done:done + batch]
+    per_pass = target_s / 12.0
+    o = _oracle_lib(True)
+    sample, t_grow, done = [], 0.0, 0
+    while done < len(order) and t_grow < per_pass:
+        chunk = order[done:done + 4]
         done += len(chunk)
-        tr2 = C.create_string_buffer(o.los_ref_sizeof_transfers2())
-        # copy the stage settings the oracle reads from the scenario's transfers2
-        C.memmove(tr2, s["ptr2"].value, o.los_ref_sizeof_transfers2())
-        k1 = (C.c_int * len(chunk))(*[p[0] for p in chunk])
-        k2 = (C.c_int * len(chunk))(*[p[1] for p in chunk])
-        n = C.c_long(0)
-        t0 = time.perf_counter()
-        rc = o.los_ref_run_pairs(s["ppr"], s["ppr2"], s["pba"], s["pth"], s["ppt2"], s["pbs"],
-                                 s["pbs2"], C.cast(tr2, C.c_void_p), k1, k2, len(chunk), C.byref(n))
-        dt = time.perf_counter() - t0
-        if rc != 0:
-            raise RuntimeError("oracle sample failed")
-        o.los_ref_transfer2_free(s["ppr2"], s["ppt2"], C.cast(tr2, C.c_void_p))
-        total_el += n.value
-        total_t += dt
+        t_grow += _oracle_pass(o, scen, chunk)[1]
+        sample += chunk
+    sample = sorted(sample)
+    n_el = sum(tt2 * sizes_of[p] for p in sample)
+    rows = np.zeros(n_el)
+    times = {}
+    for debug in (True, False):
+        lib = o if debug else _oracle_lib(False)
+        _oracle_pass(lib, scen, sample)  # warm-up
+        ts = []
+        for rep in range(5):
+            n, dt = _oracle_pass(lib, scen, sample, rows if (debug and rep == 0) else None)
+            if n != n_el:
+                raise RuntimeError(f"oracle sample: {n} elements, expected {n_el}")
+            ts.append(dt)
+        times["debug" if debug else "nodebug"] = float(np.median(ts))
+        print(f"# cpu baseline ({'DEBUG' if debug else 'no DEBUG'}): median {np.median(ts):.2f} s"
+              f" over {len(sample)} pairs", file=sys.stderr, flush=True)
     scen.close()
-    return {"value": total_el / total_t, "unit": "elements/s", "cores": threads, "kind": "port",
-            "sample": f"{done} of {len(pairs)} (k1,k2) pairs of the same workload "
-                      f"({total_el} elements, {total_t:.1f} s, OpenMP static over k3 per pair)"}
+    ref, off = {}, 0
+    for p in sample:
+        k = sizes_of[p]
+        ref[p] = rows[off:off + tt2 * k].reshape(tt2, k)
+        off += tt2 * k
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    rec = {"value": n_el / times["nodebug"], "unit": "elements/s", "cores": threads,
+           "kind": "port",
+           "value_debug": n_el / times["debug"],
+           "median_s": times, "cpu": cpu_info(),
+           "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
+                                                   "OMP_PLACES")},
+           "sample": (f"{len(sample)} of {len(pairs)} (k1,k2) pairs of the same workload, "
+                      f"{n_el} elements; oracle built gcc -O3 -fopenmp (value: without "
+                      f"-DDEBUG, value_debug: with); 1 warm-up + median of 5 passes; "
+                      "OpenMP static over k3 per pair")}
+    return rec, ref
 
 
 def relative_errors(got, ref, sizes, tt2_size):
@@ -528,7 +610,8 @@ def main(argv=None):
                          "against the fp64 form ('' for none)")
     ap.add_argument("--graph", choices=["auto", "on", "off"], default="auto",
                     help="replay each step as one captured hipGraph (auto: batched workloads)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=24.0,
+                    help="CPU baseline: about this many seconds of oracle passes in all")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-jbuild", action="store_true",
                     help="skip timing the J-table builds (host and GPU)")
@@ -540,6 +623,10 @@ def main(argv=None):
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="c5: arrival span per rate, seconds")
     args = ap.parse_args(argv)
+    # CPU-baseline protocol: threads bound close, one per core (read by the OpenMP runtime
+    # when it starts, which is when the first library that uses it loads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.gpus != 1:
         print(f"bench.py: this build runs the LOS stage on one GPU; --gpus {args.gpus} is not "
               "supported (the (k1,k2) pairs would shard across GPUs with no communication)",
@@ -625,16 +712,22 @@ def main(argv=None):
                              {"kernel": "k-lane", "l_per_lane": ch} if kind == 2 else
                              {"kernel": "workgroup per k3"} if form == "fp64" else
                              {"kernel": "stream", "nk": nk, "ch": ch, "persistent": pers})
-        return res, out, tt2
+        return res, out, tt2, tt_out
 
     forms = [args.form] + [f for f in args.also.split(",") if f and f != args.form]
     if "fp64" not in forms:
         forms.append("fp64")
-    results, outs = {}, {}
+    results, outs, full = {}, {}, {}
     tt2 = 0
     for f in forms:
-        results[f], outs[f], tt2 = run_form(f)
+        results[f], outs[f], tt2, full[f] = run_form(f)
     eng.close()
+    offs = scen.pair_offsets()
+    total_k3 = sum(n for _, n in offs.values())
+
+    def rows_of(flat, p):
+        o, n = offs[p]
+        return flat.reshape(-1, total_k3)[:, o:o + n]
 
     per_lm = {}
     for f in forms:
@@ -651,13 +744,61 @@ def main(argv=None):
                      "rms": np.sqrt(np.mean(field ** 2, axis=1)).tolist(),
                      "row_max": row.max(axis=1).tolist()}
     scen.close()
+
+    cpu = parity = None
+    per_lm_oracle = {}
+    if not args.no_cpu:
+        cpu, oref = cpu_baseline(cfg, pairs, args.cpu_seconds, tt2,
+                                 {p: offs[p][1] for p in pairs})
+        sample = sorted(oref)
+        r = np.concatenate([oref[p] for p in sample], axis=1)
+        g64 = np.concatenate([rows_of(full["fp64"], p) for p in sample], axis=1)
+        parity = {"pairs": len(sample), "compared": int(r.size),
+                  "fp64_mismatches": int(np.count_nonzero(g64 != r)), "forms": {}}
+        fscale = np.abs(r).max(axis=1, keepdims=True)
+        for f in forms:
+            if f == "fp64":
+                continue
+            g = np.concatenate([rows_of(full[f], p) for p in sample], axis=1)
+            d = np.abs(g - r)
+            fe = np.divide(d, fscale, out=np.zeros_like(d), where=fscale > 0)
+            re_ = []
+            for p in sample:
+                rr = oref[p]
+                sc = np.abs(rr).max(axis=1, keepdims=True)
+                dd = np.abs(rows_of(full[f], p) - rr)
+                re_.append(np.divide(dd, sc, out=np.zeros_like(dd), where=sc > 0))
+            re_ = np.concatenate(re_, axis=1)
+            parity["forms"][f] = {"max_rel_err": float(fe.max()),
+                                  "rms_rel_err": float(np.sqrt(np.mean(fe ** 2))),
+                                  "max_rel_err_worst_tt": int(np.argmax(fe.max(axis=1))),
+                                  "max_row_rel_err": float(re_.max())}
+            per_lm_oracle[f] = {"max": fe.max(axis=1).tolist(),
+                                "rms": np.sqrt(np.mean(fe ** 2, axis=1)).tolist(),
+                                "row_max": re_.max(axis=1).tolist()}
+        print(f"# parity vs the oracle on {len(sample)} pairs: {parity['fp64_mismatches']} of "
+              f"{r.size} fp64 values differ; " +
+              "; ".join(f"{f} max {v['max_rel_err']:.2e} rms {v['rms_rel_err']:.2e}"
+                        for f, v in parity["forms"].items()), file=sys.stderr, flush=True)
     if args.errors_out:
         with open(args.errors_out, "w") as fh:
-            json.dump({"workload": args.workload, "per_tt": per_lm, "tt2_size": tt2}, fh)
+            json.dump({"workload": args.workload, "tt2_size": tt2,
+                       "lm_of_tt": "tt = field * n_lm + lm, fields T, E, B, (l,m) l-major",
+                       "vs_fp64_gpu": per_lm, "vs_oracle": per_lm_oracle,
+                       "oracle_pairs": parity["pairs"] if parity else 0}, fh)
 
     head = results[args.form]
     alg_bytes = head["alg_bytes"]
     achieved = head.get("hbm_gbs", 0.0)
+    traffic = measured_traffic(args.workload, head["kernel"])
+    valu = {"achieved": head.get("valu_tflops", 0.0), "peak": VALU_PEAK_TFLOPS[args.form],
+            "unit": "TFLOP/s",
+            "frac": head.get("valu_tflops", 0.0) / VALU_PEAK_TFLOPS[args.form],
+            "flop_alg": flop_live, "flop_dense": flop_dense}
+    hbm = {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS}
+    # the bound is whichever roofline the kernel sits closer to
+    top = valu if valu["frac"] > hbm["frac"] else hbm
     line = {
         "metric": METRIC, "value": head["value"], "unit": "elements/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -666,23 +807,21 @@ def main(argv=None):
         "data": "synthetic (seeded source model, FP64 J tables built on the host)",
         "config": {"workload": f"{args.workload}: {DESCR[args.workload]}", "form": args.form,
                    "elements": n_el, "pairs": len(pairs)},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.workload, head["kernel"]),
+        "roofline": {"bound": "valu" if top is valu else "hbm", "achieved": top["achieved"],
+                     "peak": top["peak"], "unit": top["unit"], "frac": top["frac"],
+                     "traffic": traffic,
+                     "traffic_over_alg_bytes": traffic / alg_bytes if traffic else None,
                      "kernel": head["kernel"], "kernel_ms": head["kernel_ms"],
-                     "alg_bytes": alg_bytes,
-                     "valu": {"achieved": head.get("valu_tflops", 0.0),
-                              "peak": VALU_PEAK_TFLOPS[args.form], "unit": "TFLOP/s",
-                              "frac": head.get("valu_tflops", 0.0) / VALU_PEAK_TFLOPS[args.form],
-                              "flop_alg": flop_live, "flop_dense": flop_dense}},
+                     "alg_bytes": alg_bytes, "hbm": hbm, "valu": valu},
         "forms": results,
     }
     if probe:
         line["copy_probe"] = probe
     if jtab:
         line["j_tables"] = jtab
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cfg, pairs, args.cpu_seconds)
+    if cpu:
+        line["cpu_baseline"] = cpu
+        line["parity"] = parity
     print(json.dumps(line), flush=True)
     return 0
 

@@ -276,6 +276,7 @@ int los_ref_get_k3_list(struct precision *ppr, struct precision2 *ppr2, struct p
   r_test(k3[k_tr_size - 1] > k_max_tr, ptr2->error_message,
          "bug in k list calculation, k_max_transfers2=%.17f is larger than k_max_tr=%.17f",
          k3[k_tr_size - 1], k_max_tr);
+#ifdef DEBUG
   /* DEBUG invariants of the reference (source/transfer2.c:1711-1733) */
   int physical_size = ptr2->k_physical_size_k1k2[index_k1][index_k2];
   int first_physical_index = ptr2->k_physical_start_k1k2[index_k1][index_k2];
@@ -286,6 +287,7 @@ int los_ref_get_k3_list(struct precision *ppr, struct precision2 *ppr2, struct p
            ptr2->error_message, "triangular condition not satisfied (%d,%d)", index_k1, index_k2);
   for (int i = 0; i < k_tr_size - 1; ++i)
     r_test(k3[i] >= k3[i + 1], ptr2->error_message, "k3-grid is not strictly ascending");
+#endif
   return _SUCCESS_;
 }
 
@@ -575,12 +577,14 @@ static int ref_get_time_grid(struct precision2 *ppr2, struct perturbs2 *ppt2,
     }
     for (itr = 0; itr < pw->tau_grid_size; ++itr)
       if (pw->index_tau_left[itr] == (ppt2->tau_size - 1)) pw->index_tau_left[itr] = ppt2->tau_size - 2;
+#ifdef DEBUG /* source/transfer2.c:2541-2559 */
     for (itr = 0; itr < pw->tau_grid_size - 1; ++itr)
       r_test(pw->tau_grid[itr] >= pw->tau_grid[itr + 1], ptr2->error_message,
              "integration grid is not sorted");
     r_test(pw->tau_grid_size < ppt2->tau_size, ptr2->error_message, "integration grid too coarse");
     r_test((pw->tau_grid[0] != tau_min_pt) || (pw->tau_grid[pw->tau_grid_size - 1] != tau_max_pt),
            ptr2->error_message, "limits of the time grid differ from the sources'");
+#endif
   }
 fill_delta_tau:
   for (it = 0; it < pw->tau_grid_size; ++it) pw->tau0_minus_tau[it] = ptr2->tau0 - pw->tau_grid[it];
@@ -635,7 +639,9 @@ static int ref_integrate(struct precision *ppr, struct precision2 *ppr2, struct 
   int m = ptr2->m[index_m];
   double k = pw->k;
   *integral = 0;
+#ifdef DEBUG /* source/transfer2.c:2168-2177 */
   r_test(k * pw->tau0_minus_tau[0] > pbs2->xx_max, ptr2->error_message, "not enough J's");
+#endif
   for (int it = 0; it < pw->tau_grid_size; ++it) {
     double x = k * pw->tau0_minus_tau[it];
     int index_x = (int)(x / pbs2->xx_step);
@@ -647,9 +653,13 @@ static int ref_integrate(struct precision *ppr, struct precision2 *ppr2, struct 
       int index_x_min = pbs2->index_xmin_J[index_J][iL][index_l][index_m];
       if (index_x < index_x_min) continue;
       int ixJ = index_x - index_x_min;
+#ifdef DEBUG
+      /* source/transfer2.c:2239-2246, which tests ixJ >= x_size; J[ixJ + 1] is read
+         as well, so the bound here also covers that read */
       int x_size = pbs2->x_size_J[index_J][iL][index_l][index_m];
       r_test((ixJ + 1 >= x_size) || (ixJ < 0), ptr2->error_message,
              "L=%d, l=%d, m=%d, x=%g: wrong indexing of J (%d of %d)", L, l, m, x, ixJ, x_size);
+#endif
       double J_Llm;
       double *J = &(pbs2->J_Llm_x[index_J][iL][index_l][index_m][ixJ]);
       double Jl = *J;
@@ -669,7 +679,9 @@ static int ref_integrate(struct precision *ppr, struct precision2 *ppr2, struct 
     *integral += integrand * pw->delta_tau[it];
   }
   *integral *= 0.5;
+#ifdef DEBUG /* source/transfer2.c:2332-2337 */
   r_test(isnan(*integral), ptr2->error_message, "found nan in second-order transfer function");
+#endif
   return _SUCCESS_;
 }
 
@@ -925,7 +937,9 @@ int los_ref_transfer2_free(struct precision2 *ppr2, struct perturbs2 *ppt2, stru
  * Bounded-sample timing entry for the CPU baseline: the oracle's bookkeeping
  * (its own transfers2, filled here) and then, for each listed pair, the
  * reference loop body (k3 list, k3 interpolation, OpenMP static loop over k3,
- * compute for every tt). Results go to a flat buffer instead of ptr2->transfer.
+ * compute for every tt). The results of each pair, [tt][k3], are appended to
+ * `out` in the order of the list when `out` is not NULL (room for *n_out
+ * values is the caller's business: sum over the pairs of tt2_size x k3 count).
  * Returns the number of elements computed in *n_out. Does not free sources.
  */
 long los_ref_sizeof_transfers2(void) { return (long)sizeof(struct transfers2); }
@@ -945,7 +959,7 @@ int los_ref_indices(struct precision *ppr, struct precision2 *ppr2, struct backg
 int los_ref_run_pairs(struct precision *ppr, struct precision2 *ppr2, struct background *pba,
                       struct thermo *pth, struct perturbs2 *ppt2, struct bessels *pbs,
                       struct bessels2 *pbs2, struct transfers2 *ptr2, const int *k1s,
-                      const int *k2s, int npairs, long *n_out)
+                      const int *k2s, int npairs, long *n_out, double *out)
 {
   ptr2->tau0 = pba->conformal_age;
   ptr2->tau_rec = pth->tau_rec;
@@ -1031,13 +1045,18 @@ int los_ref_run_pairs(struct precision *ppr, struct precision2 *ppr2, struct bac
       free(isk[tp]);
     }
     for (int tt = 0; tt < ptr2->tt2_size; ++tt) {
+      if (out != NULL && !abort_flag) {
+        memcpy(out + count, ptr2->transfer[tt][i1][i2],
+               (size_t)ptr2->k_size_k1k2[i1][i2] * sizeof(double));
+        count += ptr2->k_size_k1k2[i1][i2];
+      }
       free(ptr2->transfer[tt][i1][i2]);
       free(ptr2->transfer[tt][i1]);
       ptr2->transfer[tt][i1] = save[tt];
     }
     free(save);
     if (abort_flag) return _FAILURE_;
-    count += (long)ptr2->tt2_size * ptr2->k_size_k1k2[i1][i2];
+    if (out == NULL) count += (long)ptr2->tt2_size * ptr2->k_size_k1k2[i1][i2];
   }
   free(spl);
   free(isk);

@@ -79,11 +79,17 @@ def build_library(out=os.path.join(HERE, "libsong_los.so")):
 
 
 def build_oracle(out=os.path.join(ROOT, "oracle", "liblos_oracle.so")):
+    """The oracle twice, with the reference's flags (gcc -O3 -std=c99, Makefile:15-25, plus
+    the -fopenmp it leaves commented out, Makefile:36-37): with -DDEBUG (the reference's
+    default, Makefile:19: its runtime checks on; the tests load this one) and without
+    (liblos_oracle_nodebug.so, timed beside it as the CPU baseline)."""
     src = os.path.join(ROOT, "oracle", "los_ref.c")
-    tmp_out = out + ".tmp"
-    _run(["gcc", "-O3", "-std=c99", "-D_GNU_SOURCE", "-fopenmp", "-fPIC", "-shared",
-          "-Wno-format-truncation", "-o", tmp_out, src, "-lm"])
-    os.replace(tmp_out, out)
+    nodebug = out.replace(".so", "_nodebug.so")
+    for dst, flags in ((out, ["-DDEBUG"]), (nodebug, [])):
+        tmp_out = dst + ".tmp"
+        _run(["gcc", "-O3", "-std=c99", "-D_GNU_SOURCE", "-fopenmp", "-fPIC", "-shared",
+              *flags, "-Wno-format-truncation", "-o", tmp_out, src, "-lm"])
+        os.replace(tmp_out, dst)
     return out
 
 

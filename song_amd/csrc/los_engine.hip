@@ -1357,10 +1357,10 @@ extern "C" int los_engine_flops(los_engine *e, double *live, double *dense)
 
 extern "C" double los_engine_alg_bytes(los_engine *e)
 {
-  // each coarse source column once (and its spline second derivatives when the
-  // k3 interpolation is cubic), the live part of the J table once at the
-  // precision the form reads it in, each output once
-  const double src = 8.0 * e->src_total * (e->P.cubic_k3 ? 2 : 1);
+  // compulsory bytes: each coarse source value once (the spline second
+  // derivatives are derived from them and not counted), the live part of the J
+  // table once at the precision the form reads it in, each output once
+  const double src = 8.0 * e->src_total;
   const double jb = e->J_window_bytes * (e->form == LOS_FORM_FP32C ? 0.5 : 1.0);
   return src + jb + 8.0 * (double)e->tt2_size * e->F;
 }

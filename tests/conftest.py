@@ -37,6 +37,11 @@ def load_oracle():
     o.los_ref_offset_indexl_indexm.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int,
                                                C.POINTER(C.c_int), C.c_int]
     o.los_ref_offset_indexl_indexm.restype = C.c_int
+    o.los_ref_run_pairs.argtypes = [C.c_void_p] * 8 + [C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                       C.c_int, C.POINTER(C.c_long),
+                                                       C.POINTER(C.c_double)]
+    o.los_ref_run_pairs.restype = C.c_int
+    o.los_ref_sizeof_transfers2.restype = C.c_long
     return o
 
 
@@ -63,3 +68,49 @@ def field_errors(got, ref, tt2_size):
     g, r = np.asarray(got).reshape(tt2_size, -1), np.asarray(ref).reshape(tt2_size, -1)
     sc = np.abs(r).max(axis=1, keepdims=True)
     return np.divide(np.abs(g - r), sc, out=np.zeros_like(g), where=sc > 0)
+
+
+def oracle_pairs(scen, oracle, pairs):
+    """The oracle's transfers of `pairs` on a prepared scenario, {pair: [tt2_size][k3]}.
+
+    The oracle runs on a private copy of the scenario's transfers2 (its own bookkeeping:
+    l and m lists, k3 sizes) and reads the sources without freeing them. The scenario's
+    k3 counts (the product's bookkeeping) size the buffer, and the oracle's element
+    count must agree with them."""
+    import numpy as np
+    from song_amd import lib as L
+
+    offs = scen.pair_offsets()
+    tt2 = C.c_int.from_address(scen.s["ptr2"].value +
+                               L.lib_offsets()["transfers2.tt2_size"]).value
+    n_exp = sum(tt2 * offs[p][1] for p in pairs)
+    out = np.zeros(max(n_exp, 1))
+    size = oracle.los_ref_sizeof_transfers2()
+    tr2 = C.create_string_buffer(size)
+    C.memmove(tr2, scen.s["ptr2"].value, size)
+    k1 = (C.c_int * len(pairs))(*[int(p[0]) for p in pairs])
+    k2 = (C.c_int * len(pairs))(*[int(p[1]) for p in pairs])
+    n = C.c_long(0)
+    st = scen.s
+    rc = oracle.los_ref_run_pairs(st["ppr"], st["ppr2"], st["pba"], st["pth"], st["ppt2"],
+                                  st["pbs"], st["pbs2"], C.cast(tr2, C.c_void_p), k1, k2,
+                                  len(pairs), C.byref(n),
+                                  out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc != 0:
+        raise RuntimeError("oracle run_pairs failed")
+    oracle.los_ref_transfer2_free(st["ppr2"], st["ppt2"], C.cast(tr2, C.c_void_p))
+    assert n.value == n_exp, f"oracle computed {n.value} elements, the product counts {n_exp}"
+    res, o = {}, 0
+    for p in pairs:
+        k = offs[p][1]
+        res[p] = out[o:o + tt2 * k].reshape(tt2, k)
+        o += tt2 * k
+    return res
+
+
+def product_pairs(scen, flat, pairs):
+    """The rows of `pairs` in a flat gather() array, {pair: [tt2_size][k3]}."""
+    offs = scen.pair_offsets()
+    total = sum(n for _, n in offs.values())
+    tt = flat.reshape(-1, total)
+    return {p: tt[:, offs[p][0]:offs[p][0] + offs[p][1]] for p in pairs}
