@@ -41,6 +41,15 @@
 
 #include "../include/song_los_types.h"
 
+/* LOS_PLANT = n > 0 builds the oracle with deliberate mistake n, so that the pinning
+   tests can show they catch it (1: k3 spline weights a and b swapped; 2: the spline's
+   h^2/6 term as h^2/3; 3: the J skip as index_x <= index_xmin_J; 4: the flat k3
+   extrapolation fills swapped; 5: linear k3 weights swapped; 6: linear time weights
+   swapped). The oracle proper is built with LOS_PLANT = 0. */
+#ifndef LOS_PLANT
+#define LOS_PLANT 0
+#endif
+
 #define R_MIN(a, b) ((a) < (b) ? (a) : (b))
 #define R_MAX(a, b) ((a) > (b) ? (a) : (b))
 
@@ -498,6 +507,13 @@ static int ref_interpolate_sources_in_k(struct precision *ppr, struct precision2
     r_test(h == 0, ptr2->error_message, "stop to avoid division by zero");
     double b = (k_tr[it] - k_pt[ik]) / h;
     double a = 1 - b;
+#if LOS_PLANT == 1 || LOS_PLANT == 5
+    if ((LOS_PLANT == 1) == (ppr2->sources_k3_interpolation == cubic_interpolation)) {
+      const double w = a;
+      a = b;
+      b = w;
+    }
+#endif
     if (ppr2->sources_k3_interpolation == linear_interpolation) {
       for (int t = 0; t < T; t++)
         out[it * T + t] = a * S[t * k_pt_size + ik] + b * S[t * k_pt_size + ik + 1];
@@ -505,16 +521,17 @@ static int ref_interpolate_sources_in_k(struct precision *ppr, struct precision2
       for (int t = 0; t < T; t++)
         out[it * T + t] = a * S[t * k_pt_size + ik] + b * S[t * k_pt_size + ik + 1] +
                           ((a * a * a - a) * spl[t * k_pt_size + ik] +
-                           (b * b * b - b) * spl[t * k_pt_size + ik + 1]) * h * h / 6.0;
+                           (b * b * b - b) * spl[t * k_pt_size + ik + 1]) * h * h /
+                              (LOS_PLANT == 2 ? 3.0 : 6.0);
     }
   }
   if (ppr->bispectra_k3_extrapolation != no_k3_extrapolation) {
     for (int it = last + 1; it < k_tr_size; ++it)
       if (ppr->bispectra_k3_extrapolation == flat_k3_extrapolation)
-        for (int t = 0; t < T; t++) out[it * T + t] = out[last * T + t];
+        for (int t = 0; t < T; t++) out[it * T + t] = out[(LOS_PLANT == 4 ? first : last) * T + t];
     for (int it = 0; it < first; ++it)
       if (ppr->bispectra_k3_extrapolation == flat_k3_extrapolation)
-        for (int t = 0; t < T; t++) out[it * T + t] = out[first * T + t];
+        for (int t = 0; t < T; t++) out[it * T + t] = out[(LOS_PLANT == 4 ? last : first) * T + t];
   }
   return _SUCCESS_;
 }
@@ -620,6 +637,13 @@ static int ref_interpolate_sources_in_time(struct precision2 *ppr2, struct pertu
     r_test(h == 0., ptr2->error_message, "stop to avoid division by zero");
     double b = (pw->tau_grid[itr] - tau_pt[i]) / h;
     double a = 1 - b;
+#if LOS_PLANT == 6
+    {
+      const double w = a;
+      a = b;
+      b = w;
+    }
+#endif
     if (ppr2->sources_time_interpolation == linear_interpolation)
       out[itr] = a * col[i] + b * col[i + 1];
     else
@@ -651,7 +675,7 @@ static int ref_integrate(struct precision *ppr, struct precision2 *ppr2, struct 
       int L = pbs2->L[iL];
       if (abs(m) > R_MIN(L, l)) continue;
       int index_x_min = pbs2->index_xmin_J[index_J][iL][index_l][index_m];
-      if (index_x < index_x_min) continue;
+      if (LOS_PLANT == 3 ? index_x <= index_x_min : index_x < index_x_min) continue;
       int ixJ = index_x - index_x_min;
 #ifdef DEBUG
       /* source/transfer2.c:2239-2246, which tests ixJ >= x_size; J[ixJ + 1] is read
@@ -1078,6 +1102,48 @@ int los_ref_run_pairs(struct precision *ppr, struct precision2 *ppr2, struct bac
   free(ppw);
   *n_out = count;
   return _SUCCESS_;
+}
+
+/* The oracle's time grid of one k (source/transfer2.c:2366-2585), for its pinning
+   tests: grid, index_tau_left and delta_tau, at most cap points; returns the grid size,
+   or -1 (message in ptr2->error_message). */
+int los_ref_time_grid(struct precision2 *ppr2, struct perturbs2 *ppt2, struct bessels2 *pbs2,
+                      struct transfers2 *ptr2, double k, double *tau, int *left, double *dtau,
+                      int cap)
+{
+  struct transfer2_workspace w;
+  memset(&w, 0, sizeof w);
+  w.k = k;
+  /* room for every point the grid can hold: the sources' points plus one per
+     tau_step_max over the whole range */
+  double step = 0;
+  if (ptr2->tau_sampling == custom_tau_sampling) step = 2 * M_PI / k * ppr2->tau_linstep_song;
+  if (ptr2->tau_sampling == bessel_tau_sampling) step = pbs2->xx_step / k;
+  long room = 2L * ppt2->tau_size + 2;
+  if (step > 0)
+    room += (long)ceil((ppt2->tau_sampling[ppt2->tau_size - 1] - ppt2->tau_sampling[0]) / step);
+  if (room > 50000000L) return -1;
+  const long big = room;
+  w.tau_grid = malloc(big * sizeof(double));
+  w.tau0_minus_tau = malloc(big * sizeof(double));
+  w.delta_tau = malloc(big * sizeof(double));
+  w.index_tau_left = malloc(big * sizeof(int));
+  int n = -1;
+  if (w.tau_grid && w.tau0_minus_tau && w.delta_tau && w.index_tau_left &&
+      ref_get_time_grid(ppr2, ppt2, pbs2, ptr2, &w) == _SUCCESS_ && w.tau_grid_size <= cap) {
+    n = w.tau_grid_size;
+    memcpy(tau, w.tau_grid, n * sizeof(double));
+    memcpy(dtau, w.delta_tau, n * sizeof(double));
+    if (ptr2->tau_sampling == sources_tau_sampling)
+      for (int i = 0; i < n; ++i) left[i] = i;
+    else
+      memcpy(left, w.index_tau_left, n * sizeof(int));
+  }
+  free(w.tau_grid);
+  free(w.tau0_minus_tau);
+  free(w.delta_tau);
+  free(w.index_tau_left);
+  return n;
 }
 
 /* The oracle's (l,m) helpers, exported for their fixture tests. */

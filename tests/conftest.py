@@ -17,11 +17,11 @@ def pytest_configure(config):
 ORACLE_PATH = os.path.join(ROOT, "oracle", "liblos_oracle.so")
 
 
-def load_oracle():
+def load_oracle(path=ORACLE_PATH):
     """The CPU oracle library (test infrastructure; built by `python -m song_amd.build`)."""
-    if not os.path.exists(ORACLE_PATH):
+    if not os.path.exists(path):
         pytest.skip("oracle library not built")
-    o = C.CDLL(ORACLE_PATH)
+    o = C.CDLL(path)
     o.los_ref_transfer2_init.argtypes = [C.c_void_p] * 10
     o.los_ref_transfer2_init.restype = C.c_int
     o.los_ref_transfer2_free.argtypes = [C.c_void_p] * 3
