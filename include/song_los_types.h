@@ -213,7 +213,9 @@ struct bessels2 {
 };
 
 /* ---------------------------------------------------------------------------
- * struct transfers2 (include/transfer2.h:76-238): the output of the stage.
+ * struct transfers2 (include/transfer2.h:76-238): the output of the stage,
+ * every field in SONG's order and type, so that its layout is SONG's (given
+ * the CLASS constants above).
  * ------------------------------------------------------------------------- */
 struct transfers2 {
   /* transfer[index_tt][index_k1][index_k2][index_k] */
@@ -252,6 +254,8 @@ struct transfers2 {
   char transfers_dir[_FILENAMESIZE_];
   char **transfers_paths;
   FILE **transfers_files;
+  FILE *transfers_status_file;                  /* not used by SONG either */
+  char transfers_status_path[_FILENAMESIZE_];   /* not used by SONG either */
 
   double tau0;
   double tau_rec;

@@ -39,6 +39,29 @@ int los_struct_offsets(long *out, int n)
   return 3;
 }
 
+/* Byte offsets of every field of struct transfers2 in declaration order
+   (include/transfer2.h:76-238), then its size; returns the number of values. */
+int los_transfers2_layout(long *out, int n)
+{
+#define OFF(f) (long)offsetof(struct transfers2, f)
+  const long v[] = {
+      OFF(transfer), OFF(index_tt2_T), OFF(index_tt2_E), OFF(index_tt2_B), OFF(tt2_size),
+      OFF(tt2_labels), OFF(has_cls), OFF(l_size), OFF(l), OFF(m), OFF(m_size), OFF(n_transfers),
+      OFF(n_nonzero_transfers_E), OFF(n_nonzero_transfers_B), OFF(index_tt2_monopole),
+      OFF(index_pt2_monopole), OFF(corresponding_index_l), OFF(corresponding_index_m),
+      OFF(lm_array), OFF(k_sampling), OFF(tau_sampling), OFF(k_size_k1k2),
+      OFF(k_physical_start_k1k2), OFF(k_physical_size_k1k2), OFF(k_min_k1k2), OFF(k_max_k1k2),
+      OFF(k3_size_max), OFF(transfers_dir), OFF(transfers_paths), OFF(transfers_files),
+      OFF(transfers_status_file), OFF(transfers_status_path), OFF(tau0), OFF(tau_rec),
+      OFF(rs_rec), OFF(transfer2_verbose), OFF(error_message), OFF(count_allocated_transfers),
+      OFF(count_memorised_transfers), OFF(has_allocated_transfers), OFF(stop_at_transfers2),
+      (long)sizeof(struct transfers2)};
+#undef OFF
+  const int nv = (int)(sizeof v / sizeof v[0]);
+  for (int i = 0; i < n && i < nv; ++i) out[i] = v[i];
+  return nv;
+}
+
 /* ---------------------------------------------------------------------------
  * Producer-side helpers (weak: a SONG link provides its own).
  * source/perturbations2.c:457-508, :521-602, :609-636

@@ -201,3 +201,40 @@ def test_sources_files_round_trip(tmp_path):
     assert lib.perturb2_load_sources_from_disk(ppt2, i1) == 1
     r.close()
     ref.close()
+
+
+def test_transfers2_layout_is_songs():
+    """struct transfers2 has SONG's fields in SONG's order and types
+    (include/transfer2.h:76-238), so its layout equals SONG's for the same CLASS
+    constants: the offsets the library was compiled with equal those of a C struct
+    declared here field by field from the reference header."""
+    import ctypes as C
+    FN, LBL, ERR = 256, 64, 2048  # _FILENAMESIZE_, _MAX_LENGTH_LABEL_, _ERRORMSGSIZE_
+    p, i, sh, d, lg = C.c_void_p, C.c_int, C.c_short, C.c_double, C.c_long
+    fields = [("transfer", p), ("index_tt2_T", i), ("index_tt2_E", i), ("index_tt2_B", i),
+              ("tt2_size", i), ("tt2_labels", p), ("has_cls", sh), ("l_size", i), ("l", p),
+              ("m", p), ("m_size", i), ("n_transfers", i), ("n_nonzero_transfers_E", i),
+              ("n_nonzero_transfers_B", i), ("index_tt2_monopole", p),
+              ("index_pt2_monopole", p), ("corresponding_index_l", p),
+              ("corresponding_index_m", p), ("lm_array", p), ("k_sampling", i),
+              ("tau_sampling", i), ("k_size_k1k2", p), ("k_physical_start_k1k2", p),
+              ("k_physical_size_k1k2", p), ("k_min_k1k2", p), ("k_max_k1k2", p),
+              ("k3_size_max", i), ("transfers_dir", C.c_char * FN), ("transfers_paths", p),
+              ("transfers_files", p), ("transfers_status_file", p),
+              ("transfers_status_path", C.c_char * FN), ("tau0", d), ("tau_rec", d),
+              ("rs_rec", d), ("transfer2_verbose", sh), ("error_message", C.c_char * ERR),
+              ("count_allocated_transfers", lg), ("count_memorised_transfers", lg),
+              ("has_allocated_transfers", p), ("stop_at_transfers2", sh)]
+
+    class Song(C.Structure):
+        _fields_ = fields
+
+    lib = L.load()
+    fn = lib.los_transfers2_layout
+    fn.argtypes, fn.restype = [C.POINTER(C.c_long), C.c_int], C.c_int
+    buf = (C.c_long * 64)()
+    n = fn(buf, 64)
+    assert n == len(fields) + 1
+    want = [getattr(Song, f).offset for f, _ in fields] + [C.sizeof(Song)]
+    assert list(buf[:n]) == want
+    assert LBL == 64
