@@ -21,11 +21,29 @@
 #ifndef SONG_LOS_H
 #define SONG_LOS_H
 
-#include "song_los_types.h"
-
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* SONG's and CLASS's structures, by pointer only: this header compiles next to
+   SONG's own headers as well as next to song_los_types.h. */
+struct precision;
+struct precision2;
+struct background;
+struct thermo;
+struct perturbs;
+struct perturbs2;
+struct bessels;
+struct bessels2;
+struct transfers;
+struct transfers2;
+
+#ifndef _SUCCESS_
+#define _SUCCESS_ 0
+#define _FAILURE_ 1
+#endif
+
+#define LOS_M_LIST_MAX 14 /* entries of los_config.m_list (SONG's _MAX_NUM_AZIMUTHAL_) */
 
 /* ------------------------------------------------------------------------
  * Kernel forms.
@@ -201,7 +219,7 @@ typedef struct los_config {
   /* fields */
   int has_T;          /* temperature */
   int has_E;          /* E polarisation (B sources appear when m_max > 0) */
-  int m_list[_MAX_NUM_AZIMUTHAL_];
+  int m_list[LOS_M_LIST_MAX];
   int m_size;
   int l_max_los_t;    /* sources L = 0..l_max_los_t for T */
   int l_max_los_p;    /* and for E, B */

@@ -24,8 +24,10 @@ extern "C" {
 
 /* CLASS conventions (common.h of the CLASS fork): functions return _SUCCESS_
    or _FAILURE_ and leave a message in an ErrorMsg buffer. */
+#ifndef _SUCCESS_
 #define _SUCCESS_ 0
 #define _FAILURE_ 1
+#endif
 #define _TRUE_ 1
 #define _FALSE_ 0
 #define _ERRORMSGSIZE_ 2048

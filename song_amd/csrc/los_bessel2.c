@@ -369,3 +369,51 @@ int los_bessel2_free(struct bessels *pbs, struct bessels2 *pbs2)
   pbs2->ddJ_Llm_x = NULL;
   return _SUCCESS_;
 }
+
+/* ---------------------------------------------------------------------------
+ * The tables built on the GPU (los_bessel2_gpu_fill on the flat los_jbuild
+ * inputs): the setup and the row allocation stay here, with the structures.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  struct bessels2 *pbs2;
+} gpu_rows_ctx;
+
+static int gpu_alloc_rows(void *ctx, const los_jrow *rows, int nrows, double **dst, char *errmsg)
+{
+  struct bessels2 *pbs2 = ((gpu_rows_ctx *)ctx)->pbs2;
+  if (los_bessel2_alloc_rows(pbs2, rows, nrows, errmsg) == _FAILURE_) return _FAILURE_;
+  for (int q = 0; q < nrows; ++q)
+    dst[q] = pbs2->J_Llm_x[rows[q].J][rows[q].iL][rows[q].il][rows[q].im];
+  return _SUCCESS_;
+}
+
+int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                          struct bessels *pbs, struct bessels2 *pbs2, int device, char *errmsg)
+{
+  los_jrow *rows = NULL;
+  int nrows = 0, l1max = 0;
+  if (los_bessel2_setup(ppr, ppr2, ppt2, pbs, pbs2, &rows, &nrows, &l1max, errmsg) == _FAILURE_)
+    return _FAILURE_;
+  gpu_rows_ctx ctx = {pbs2};
+  los_jbuild d;
+  d.device = device;
+  d.nx = pbs2->xx_size;
+  d.xx = pbs2->xx;
+  d.xx_step = pbs2->xx_step;
+  d.xx_max = pbs2->xx_max;
+  d.j_cut = pbs2->j_l1_cut;
+  d.J_cut = pbs2->J_Llm_cut;
+  d.L_size = pbs2->L_size;
+  d.L = pbs2->L;
+  d.l_size = pbs->l_size;
+  d.l = pbs->l;
+  d.rows = rows;
+  d.nrows = nrows;
+  d.l1max = l1max;
+  d.alloc_rows = gpu_alloc_rows;
+  d.ctx = &ctx;
+  const int rc = los_bessel2_gpu_fill(&d, errmsg);
+  if (rc == _SUCCESS_) los_bessel2_spline_rows(pbs, pbs2, rows, nrows);
+  free(rows);
+  return rc;
+}

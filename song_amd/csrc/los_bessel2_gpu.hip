@@ -33,7 +33,7 @@
 #include <thread>
 #include <vector>
 
-#include "los_internal.h"
+#include "los_common.h"
 
 #define BCHK(call)                                                                           \
   do {                                                                                       \
@@ -223,15 +223,12 @@ los_jrow_fill(const los_jrow *__restrict__ rows, const int *__restrict__ order,
     o[ix - r.xmin] = los_jrow_value(r, jl, nx, xmin_l1, ix);
 }
 
-extern "C" int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *ppr2,
-                                     struct perturbs2 *ppt2, struct bessels *pbs,
-                                     struct bessels2 *pbs2, int device, char *errmsg)
+extern "C" int los_bessel2_gpu_fill(const los_jbuild *d, char *errmsg)
 {
-  los_jrow *rows = nullptr;
-  int nrows = 0, l1max = 0;
-  if (los_bessel2_setup(ppr, ppr2, ppt2, pbs, pbs2, &rows, &nrows, &l1max, errmsg) == _FAILURE_)
-    return _FAILURE_;
-  const int nx = pbs2->xx_size;
+  los_jrow *rows = d->rows;
+  const int nrows = d->nrows, l1max = d->l1max, device = d->device;
+  const int nx = d->nx;
+  std::vector<double *> dst(nrows > 0 ? nrows : 1, nullptr);
   double *dxx = nullptr, *djl = nullptr, *dout = nullptr;
   int *dxmin = nullptr, *dL = nullptr, *dl = nullptr, *dorder = nullptr;
   los_jrow *drows = nullptr;
@@ -249,27 +246,27 @@ extern "C" int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *p
   BCHK(hipMalloc((void **)&djl, (size_t)(l1max + 1) * nx * sizeof(double)));
   BCHK(hipMalloc((void **)&dxmin, (size_t)(l1max + 1) * sizeof(int)));
   BCHK(hipMalloc((void **)&drows, (size_t)(nrows > 0 ? nrows : 1) * sizeof(los_jrow)));
-  BCHK(hipMalloc((void **)&dL, (size_t)pbs2->L_size * sizeof(int)));
-  BCHK(hipMalloc((void **)&dl, (size_t)pbs->l_size * sizeof(int)));
-  BCHK(hipMemcpyAsync(dxx, pbs2->xx, (size_t)nx * sizeof(double), hipMemcpyHostToDevice, s));
+  BCHK(hipMalloc((void **)&dL, (size_t)d->L_size * sizeof(int)));
+  BCHK(hipMalloc((void **)&dl, (size_t)d->l_size * sizeof(int)));
+  BCHK(hipMemcpyAsync(dxx, d->xx, (size_t)nx * sizeof(double), hipMemcpyHostToDevice, s));
   BCHK(hipMemcpyAsync(drows, rows, (size_t)nrows * sizeof(los_jrow), hipMemcpyHostToDevice, s));
-  BCHK(hipMemcpyAsync(dL, pbs2->L, (size_t)pbs2->L_size * sizeof(int), hipMemcpyHostToDevice, s));
-  BCHK(hipMemcpyAsync(dl, pbs->l, (size_t)pbs->l_size * sizeof(int), hipMemcpyHostToDevice, s));
+  BCHK(hipMemcpyAsync(dL, d->L, (size_t)d->L_size * sizeof(int), hipMemcpyHostToDevice, s));
+  BCHK(hipMemcpyAsync(dl, d->l, (size_t)d->l_size * sizeof(int), hipMemcpyHostToDevice, s));
 
   hipLaunchKernelGGL(los_jl_table, dim3((nx + 63) / 64), dim3(64), 0, s, l1max, dxx, nx, djl);
   BCHK(hipGetLastError());
   hipLaunchKernelGGL(los_jl_xmin, dim3((l1max + 64) / 64), dim3(64), 0, s, l1max, dxx, nx,
-                     pbs2->xx_step, pbs2->j_l1_cut, djl, dxmin);
+                     d->xx_step, d->j_cut, djl, dxmin);
   BCHK(hipGetLastError());
   if (nrows > 0) {
     hipLaunchKernelGGL(los_jrow_xmin, dim3((nrows + 63) / 64), dim3(64), 0, s, drows, nrows, dL,
-                       dl, dxx, nx, pbs2->xx_max, pbs2->J_Llm_cut, djl, dxmin);
+                       dl, dxx, nx, d->xx_max, d->J_cut, djl, dxmin);
     BCHK(hipGetLastError());
     BCHK(hipMemcpyAsync(rows, drows, (size_t)nrows * sizeof(los_jrow), hipMemcpyDeviceToHost, s));
   }
   BCHK(hipStreamSynchronize(s));
 
-  if (los_bessel2_alloc_rows(pbs2, rows, nrows, errmsg) == _FAILURE_) goto fail_quiet;
+  if (d->alloc_rows(d->ctx, rows, nrows, dst.data(), errmsg) == _FAILURE_) goto fail_quiet;
   for (int q = 0; q < nrows; ++q) off[q + 1] = off[q] + (nx - rows[q].xmin);
   if (nrows > 0 && off[nrows] > 0) {
     BCHK(hipMalloc((void **)&dout, (size_t)off[nrows] * sizeof(double)));
@@ -280,7 +277,7 @@ extern "C" int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *p
     // rows l-major (stable in the setup order within one l)
     {
       int n = 0;
-      for (int il = 0; il < pbs->l_size; ++il)
+      for (int il = 0; il < d->l_size; ++il)
         for (int q = 0; q < nrows; ++q)
           if (rows[q].il == il) order[n++] = q;
     }
@@ -326,9 +323,7 @@ extern "C" int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *p
           long q = std::upper_bound(off.begin(), off.end(), a) - off.begin() - 1;
           for (long i = a; i < z && q < nrows;) {
             const long e = std::min(z, off[q + 1]);
-            const los_jrow &r = rows[q];
-            memcpy(pbs2->J_Llm_x[r.J][r.iL][r.il][r.im] + (i - off[q]), src + (i - c0),
-                   (size_t)(e - i) * sizeof(double));
+            memcpy(dst[q] + (i - off[q]), src + (i - c0), (size_t)(e - i) * sizeof(double));
             i = e;
             ++q;
           }
@@ -336,7 +331,6 @@ extern "C" int los_bessel2_build_gpu(struct precision *ppr, struct precision2 *p
       for (auto &x : th) x.join();
     }
   }
-  los_bessel2_spline_rows(pbs, pbs2, rows, nrows);
   rc = _SUCCESS_;
   goto done;
 fail:
@@ -358,6 +352,5 @@ done:
     if (ev[b]) (void)hipEventDestroy(ev[b]);
   }
   if (s) (void)hipStreamDestroy(s);
-  free(rows);
   return rc;
 }

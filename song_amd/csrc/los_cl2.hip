@@ -31,7 +31,7 @@
 #include <cstring>
 #include <vector>
 
-#include "los_internal.h"
+#include "los_common.h"
 
 #define CL2_THREADS 256
 

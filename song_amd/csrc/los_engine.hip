@@ -21,7 +21,7 @@
 #include <vector>
 
 #include "los_device.h"
-#include "los_internal.h"
+#include "los_common.h"
 
 #include "los_engine_impl.h"
 
@@ -178,51 +178,36 @@ extern "C" double los_engine_batch_budget(los_engine *e)
   return 0.6 * (double)fr;
 }
 
-extern "C" double los_engine_batch_bytes(struct precision2 *ppr2, struct perturbs2 *ppt2,
-                                         struct transfers2 *ptr2, int k1_begin, int k1_end)
-{
-  (void)ppr2;
-  double b = 0;
-  for (int i1 = k1_begin; i1 < k1_end; ++i1)
-    for (int i2 = 0; i2 <= i1; ++i2) {
-      b += 2.0 * 8.0 * ppt2->tp2_size * (double)ppt2->tau_size * ppt2->k3_size[i1][i2];
-      b += 8.0 * (ptr2->tt2_size + 4.0) * ptr2->k_size_k1k2[i1][i2];
-    }
-  return b;
-}
-
 // Build the device projection table [jt][L][x][lm] for x < X.
-static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2 *ppt2,
-                        struct bessels2 *pbs2, struct transfers2 *ptr2, double kmax_all,
-                        char *err)
+static int build_tables(los_engine *e, const los_stage *st, double kmax_all, char *err)
 {
-  const int J_size = pbs2->J_size, L_size = pbs2->L_size, n_lm = ptr2->n_transfers;
-  const double t0mt0 = ptr2->tau0 - ppt2->tau_sampling[0];
+  const int J_size = st->J_size, L_size = st->L_size, n_lm = st->n_transfers;
+  const double t0mt0 = st->tau0 - st->tau[0];
   const double xmax = kmax_all * t0mt0;
-  const int ixmax = (int)(xmax / pbs2->xx_step);
-  if (ixmax + 1 > pbs2->xx_size - 1) {
+  const int ixmax = (int)(xmax / st->xx_step);
+  if (ixmax + 1 > st->xx_size - 1) {
     snprintf(err, _ERRORMSGSIZE_,
              "los_engine_load: not enough J's: k*(tau0-tau_min)=%g exceeds the x grid (xx_max=%g)",
-             xmax, pbs2->xx_max);
+             xmax, st->xx_max);
     return _FAILURE_;
   }
-  const int X = std::min(ixmax + 2, pbs2->xx_size);
+  const int X = std::min(ixmax + 2, st->xx_size);
   const size_t nJ = (size_t)J_size * L_size * X * n_lm;
   std::vector<double> hJ(nJ, 0.0);
   std::vector<int> hxmin((size_t)J_size * L_size * n_lm, INT_MAX);
   double window = 0;
   for (int jt = 0; jt < J_size; ++jt)
     for (int L = 0; L < L_size; ++L)
-      for (int il = 0; il < ptr2->l_size; ++il) {
-        const int l = ptr2->l[il];
-        const int mcount = std::min(ppr2->index_m_max[L], ppr2->index_m_max[l]) + 1;
+      for (int il = 0; il < st->l_size; ++il) {
+        const int l = st->l[il];
+        const int mcount = std::min(st->index_m_max[L], st->index_m_max[l]) + 1;
         for (int im = 0; im < mcount; ++im) {
-          const int lm = ptr2->lm_array[il][im];
-          const int xm = pbs2->index_xmin_J[jt][L][il][im];
-          const int xs = pbs2->x_size_J[jt][L][il][im];
+          const int lm = st->lm_array[il][im];
+          const int xm = st->xmin_J[jt][L][il][im];
+          const int xs = st->x_size_J[jt][L][il][im];
           hxmin[((size_t)jt * L_size + L) * n_lm + lm] = xm;
           if (xm >= X) continue;
-          const double *row = pbs2->J_Llm_x[jt][L][il][im];
+          const double *row = st->J[jt][L][il][im];
           const int xend = std::min(X, xm + xs);
           for (int x = xm; x < xend; ++x)
             hJ[(((size_t)jt * L_size + L) * X + x) * n_lm + lm] = row[x - xm];
@@ -231,37 +216,36 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
       }
   // smallest first-live index over the rows each (l,m) element uses
   std::vector<int> hminx(n_lm, INT_MAX), hl(n_lm), hm(n_lm), hmi(n_lm);
-  for (int il = 0; il < ptr2->l_size; ++il)
-    for (int im = 0; im <= ppr2->index_m_max[ptr2->l[il]]; ++im) {
-      const int lm = ptr2->lm_array[il][im];
-      hl[lm] = ptr2->l[il];
-      hm[lm] = ptr2->m[im];
+  for (int il = 0; il < st->l_size; ++il)
+    for (int im = 0; im <= st->index_m_max[st->l[il]]; ++im) {
+      const int lm = st->lm_array[il][im];
+      hl[lm] = st->l[il];
+      hm[lm] = st->m[im];
       hmi[lm] = im;
       int mn = INT_MAX;
       auto scan = [&](int jt, int Lmax) {
         if (jt < 0) return;
         for (int L = 0; L <= Lmax && L < L_size; ++L) {
-          if (ptr2->m[im] > L) continue;
+          if (st->m[im] > L) continue;
           mn = std::min(mn, hxmin[((size_t)jt * L_size + L) * n_lm + lm]);
         }
       };
       // rows an element reads: TT for T; EE for E and B; EB for B (any m) and
       // for E when m != 0
-      if (ppt2->has_source_T) scan(pbs2->index_J_TT, ppr2->l_max_los_t);
-      if (ppt2->has_source_E || ppt2->has_source_B) {
-        scan(pbs2->index_J_EE, ppr2->l_max_los_p);
-        if (pbs2->has_J_EB && (ppt2->has_source_B || ptr2->m[im] != 0))
-          scan(pbs2->index_J_EB, ppr2->l_max_los_p);
+      if (st->has_T) scan(st->jTT, st->LmaxT);
+      if (st->has_E || st->has_B) {
+        scan(st->jEE, st->LmaxP);
+        if (st->jEB >= 0 && (st->has_B || st->m[im] != 0)) scan(st->jEB, st->LmaxP);
       }
       hminx[lm] = mn;
     }
   // sweep layout: slot lp = mi * n_lp + il for every (l, m-list entry)
-  const int m_size = ppr2->m_size, n_l = ptr2->l_size;
+  const int m_size = st->m_size, n_l = st->l_size;
   const int n_lp = (n_l + 63) / 64 * 64, NLP = m_size * n_lp;
   std::vector<int> lm_of_lp(NLP, -1);
   for (int il = 0; il < n_l; ++il)
     for (int im = 0; im < m_size; ++im)
-      if (im <= ppr2->index_m_max[ptr2->l[il]]) lm_of_lp[im * n_lp + il] = ptr2->lm_array[il][im];
+      if (im <= st->index_m_max[st->l[il]]) lm_of_lp[im * n_lp + il] = st->lm_array[il][im];
   const size_t nJp = (size_t)J_size * L_size * X * NLP;
   std::vector<double> hJp(nJp, 0.0);
   std::vector<int> hxminp((size_t)J_size * L_size * NLP, INT_MAX);
@@ -301,7 +285,7 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
       for (int im = 0; im < m_size; ++im) {
         std::vector<int> lms;
         for (int il = 0; il < n_l; ++il)
-          if (im <= ppr2->index_m_max[ptr2->l[il]]) lms.push_back(ptr2->lm_array[il][im]);
+          if (im <= st->index_m_max[st->l[il]]) lms.push_back(st->lm_array[il][im]);
         for (size_t q = 0; q < lms.size(); q += nlb)
           for (int j = 0; j < nlb; ++j) grp.push_back(q + j < lms.size() ? lms[q + j] : -1);
       }
@@ -322,14 +306,14 @@ static int build_tables(los_engine *e, struct precision2 *ppr2, struct perturbs2
   if (upload_vec(e, e->dl_of, hl, err)) return _FAILURE_;
   if (upload_vec(e, e->dm_of, hm, err)) return _FAILURE_;
   if (upload_vec(e, e->dmi_of, hmi, err)) return _FAILURE_;
-  std::vector<double> hxx(pbs2->xx, pbs2->xx + X);
+  std::vector<double> hxx(st->xx, st->xx + X);
   if (upload_vec(e, e->dxx, hxx, err)) return _FAILURE_;
   e->X = X;
   e->L_size = L_size;
   e->J_size = J_size;
   e->n_lm = n_lm;
   e->nJ = nJ;
-  e->J_owner = pbs2;
+  e->J_owner = st->J_id;
   e->J_window_bytes = window;
   return _SUCCESS_;
 }
@@ -368,37 +352,34 @@ std::vector<los_tile> make_tiles(const std::vector<int> &job_end, int nk)
   return t;
 }
 
-extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
-                                     struct precision2 *ppr2, struct perturbs2 *ppt2,
-                                     struct bessels *pbs, struct bessels2 *pbs2,
-                                     struct transfers2 *ptr2, const int *pk1, const int *pk2,
-                                     int npairs, char *err)
+extern "C" int los_engine_load_stage(los_engine *e, const los_stage *st, const int *pk1,
+                                     const int *pk2, int npairs, char *err)
 {
   HIPCHK(hipSetDevice(e->device), err);
-  if (ptr2->tau_sampling != sources_tau_sampling) {
+  if (!st->tau_sources) {
     snprintf(err, _ERRORMSGSIZE_,
              "los_engine_load: only transfer2_tau_sampling=sources is supported by this build");
     return _FAILURE_;
   }
-  if (ppr->bessels_interpolation != linear_interpolation) {
+  if (!st->linear_J) {
     snprintf(err, _ERRORMSGSIZE_,
              "los_engine_load: only linear bessels_interpolation is supported by this build");
     return _FAILURE_;
   }
-  if (ppt2->tp2_size <= 0 || ppt2->tau_size < 2) {
+  if (st->tp2_size <= 0 || st->tau_size < 2) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_load: empty source table");
     return _FAILURE_;
   }
-  const int LmaxT = ppr2->l_max_los_t, LmaxP = ppr2->l_max_los_p;
-  if (std::max(LmaxT, LmaxP) >= LOS_MAX_L || ppr2->m_size > LOS_MAX_MI) {
+  const int LmaxT = st->LmaxT, LmaxP = st->LmaxP;
+  if (std::max(LmaxT, LmaxP) >= LOS_MAX_L || st->m_size > LOS_MAX_MI) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_load: l_max_los (%d) or m_size (%d) too large",
-             std::max(LmaxT, LmaxP), ppr2->m_size);
+             std::max(LmaxT, LmaxP), st->m_size);
     return _FAILURE_;
   }
   for (int q = 0; q < npairs; ++q)
-    if (pk1[q] < 0 || pk1[q] >= ppt2->k_size || pk2[q] < 0 || pk2[q] > pk1[q]) {
+    if (pk1[q] < 0 || pk1[q] >= st->k_size || pk2[q] < 0 || pk2[q] > pk1[q]) {
       snprintf(err, _ERRORMSGSIZE_, "los_engine_load_pairs: pair %d = (%d,%d) is not a (k1,k2) "
-               "pair with 0 <= index_k2 <= index_k1 < %d", q, pk1[q], pk2[q], ppt2->k_size);
+               "pair with 0 <= index_k2 <= index_k1 < %d", q, pk1[q], pk2[q], st->k_size);
       return _FAILURE_;
     }
   free_batch(e);
@@ -408,19 +389,20 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
     e->pairs.clear();
     e->F = 0;
     e->n_chunk = 1;
-    e->tt2_size = ptr2->tt2_size;
+    e->tt2_size = st->tt2_size;
     e->loaded = true;
     return _SUCCESS_;
   }
-  const int ntau = ppt2->tau_size;
-  const int tp2 = ppt2->tp2_size;
+  const int ntau = st->tau_size;
+  const int tp2 = st->tp2_size;
 
   // time grid for transfer2_tau_sampling == sources (source/transfer2.c:2405-2412, :2567-2580)
   std::vector<double> t0mt(ntau), dtau(ntau);
-  for (int i = 0; i < ntau; ++i) t0mt[i] = ptr2->tau0 - ppt2->tau_sampling[i];
-  dtau[0] = ppt2->tau_sampling[1] - ppt2->tau_sampling[0];
-  for (int i = 1; i < ntau - 1; ++i) dtau[i] = ppt2->tau_sampling[i + 1] - ppt2->tau_sampling[i - 1];
-  dtau[ntau - 1] = ppt2->tau_sampling[ntau - 1] - ppt2->tau_sampling[ntau - 2];
+  const double *tau = st->tau;
+  for (int i = 0; i < ntau; ++i) t0mt[i] = st->tau0 - tau[i];
+  dtau[0] = tau[1] - tau[0];
+  for (int i = 1; i < ntau - 1; ++i) dtau[i] = tau[i + 1] - tau[i - 1];
+  dtau[ntau - 1] = tau[ntau - 1] - tau[ntau - 2];
 
   // pairs, fine k3 grids and their brackets in the coarse grid
   e->pairs.clear();
@@ -428,36 +410,33 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   std::vector<double> kc, kf, spl;
   std::vector<int> fpair, fsrc, fbr;
   long src_off = 0;
-  std::vector<double> klist(std::max(1, ptr2->k3_size_max));
+  std::vector<double> klist(std::max(1, st->k3_size_max));
   for (int q = 0; q < npairs; ++q) {
     {
       const int i1 = pk1[q], i2 = pk2[q];
-      if (!ppt2->has_allocated_sources[i1]) {
+      if (!st->has_sources[i1]) {
         snprintf(err, _ERRORMSGSIZE_, "los_engine_load: sources of index_k1=%d are not allocated", i1);
         return _FAILURE_;
       }
       los_pair p;
       p.k1 = i1;
       p.k2 = i2;
-      p.nc = ppt2->k3_size[i1][i2];
+      p.nc = st->k3_size[i1][i2];
       p.kc_off = (int)kc.size();
       p.src_off = src_off;
       p.f_off = (int)kf.size();
-      p.nf = ptr2->k_size_k1k2[i1][i2];
+      p.nf = st->k_size_k1k2[i1][i2];
       if (p.nc > LOS_NC_MAX || p.nc < 4) {
         snprintf(err, _ERRORMSGSIZE_, "los_engine_load: coarse k3 count %d outside [4,%d]", p.nc,
                  LOS_NC_MAX);
         return _FAILURE_;
       }
       int last_used = 0;
-      if (transfer2_get_k3_list(ppr, ppr2, ppt2, pbs, pbs2, ptr2, i1, i2, klist.data(),
-                                &last_used) == _FAILURE_) {
-        memcpy(err, ptr2->error_message, _ERRORMSGSIZE_);
-        return _FAILURE_;
-      }
-      const double *kpt = ppt2->k3[i1][i2];
-      const int first = ptr2->k_physical_start_k1k2[i1][i2];
-      const int last = first + ptr2->k_physical_size_k1k2[i1][i2] - 1;
+      (void)last_used;
+      if (st->k3_list(st, i1, i2, klist.data(), err) != _SUCCESS_) return _FAILURE_;
+      const double *kpt = st->k3[i1][i2];
+      const int first = st->k_phys_start[i1][i2];
+      const int last = first + st->k_phys_size[i1][i2] - 1;
       // bracket walk of source/transfer2.c:2681-2686
       std::vector<int> br(p.nf, 0);
       int ik = 0;
@@ -535,15 +514,15 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   e->ht0mt = t0mt;
   e->hforder = forder;
   e->hkf = kf;
-  e->tt2_size = ptr2->tt2_size;
+  e->tt2_size = st->tt2_size;
   e->ntau = ntau;
 
   // projection tables: built once for the whole k range of the run
-  if (e->J_owner != (const void *)pbs2) {
+  if (e->J_owner != st->J_id) {
     double kmax_all = 0;
-    for (int i1 = 0; i1 < ppt2->k_size; ++i1)
-      for (int i2 = 0; i2 <= i1; ++i2) kmax_all = std::max(kmax_all, ptr2->k_max_k1k2[i1][i2]);
-    if (build_tables(e, ppr2, ppt2, pbs2, ptr2, kmax_all, err) == _FAILURE_) return _FAILURE_;
+    for (int i1 = 0; i1 < st->k_size; ++i1)
+      for (int i2 = 0; i2 <= i1; ++i2) kmax_all = std::max(kmax_all, st->k_max_k1k2[i1][i2]);
+    if (build_tables(e, st, kmax_all, err) == _FAILURE_) return _FAILURE_;
   }
 
   // estimated cost of each sorted position: the number of time steps before
@@ -555,7 +534,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
     int lo = 0, hi = ntau;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
-      if ((int)(k * t0mt[mid] / pbs2->xx_step) >= e->minx_all) lo = mid + 1; else hi = mid;
+      if ((int)(k * t0mt[mid] / st->xx_step) >= e->minx_all) lo = mid + 1; else hi = mid;
     }
     e->pos_cost[sp] = lo + 16.0;
   }
@@ -567,7 +546,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
       int lo = 0, hi = ntau;
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if ((int)(k * t0mt[mid] / pbs2->xx_step) >= mx) lo = mid + 1; else hi = mid;
+        if ((int)(k * t0mt[mid] / st->xx_step) >= mx) lo = mid + 1; else hi = mid;
       }
       return lo;
     };
@@ -589,9 +568,9 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   jobs[0].obase = 0;
   jobs[0].ostride = e->F;
   jobs[0].n_lm = e->n_lm;
-  jobs[0].ttT = ptr2->index_tt2_T;
-  jobs[0].ttE = ptr2->index_tt2_E;
-  jobs[0].ttB = ptr2->index_tt2_B;
+  jobs[0].ttT = st->index_tt2_T;
+  jobs[0].ttE = st->index_tt2_E;
+  jobs[0].ttB = st->index_tt2_B;
   for (int i = 0; i < 5; ++i) {
     // tiles never straddle two chunks; chunk_tile[i] holds each chunk's first tile
     e->htiles[i] = make_tiles(std::vector<int>(e->chunk_pos.begin() + 1, e->chunk_pos.end()),
@@ -624,7 +603,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   for (const los_pair &p : e->pairs)
     for (int tp = 0; tp < tp2; ++tp) {
       const size_t n = (size_t)ntau * p.nc;
-      if (upload(e, e->draw + p.src_off + (long)tp * n, ppt2->sources[tp][p.k1][p.k2],
+      if (upload(e, e->draw + p.src_off + (long)tp * n, st->sources[tp][p.k1][p.k2],
                  n * sizeof(double), err))
         return _FAILURE_;
     }
@@ -655,7 +634,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   P.n_lp = e->n_lp;
   P.n_wave = e->n_wave;
   P.m_size = e->m_size;
-  for (int im = 0; im < LOS_MAX_MI; ++im) P.m_of_mi[im] = im < ppr2->m_size ? ppr2->m[im] : -1;
+  for (int im = 0; im < LOS_MAX_MI; ++im) P.m_of_mi[im] = im < st->m_size ? st->m[im] : -1;
   P.tau_end_fw = e->dtau_end_fw;
   {
     std::vector<int> rank(e->F);
@@ -672,7 +651,7 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   P.dd = e->ddd;
   P.F = e->F;
   P.tp2_size = tp2;
-  P.cubic_k3 = ppr2->sources_k3_interpolation == cubic_interpolation;
+  P.cubic_k3 = st->cubic_k3;
   P.t0mt = e->dt0mt;
   P.dtau = e->ddtau;
   P.ntau = ntau;
@@ -680,33 +659,33 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   P.xmin = e->dxmin;
   P.min_xmin = e->dminx;
   P.xx = e->dxx;
-  P.xx_step = pbs2->xx_step;
+  P.xx_step = st->xx_step;
   P.X = e->X;
   P.L_size = e->L_size;
   P.n_lm = e->n_lm;
   P.l_of = e->dl_of;
   P.m_of = e->dm_of;
   P.mi_of = e->dmi_of;
-  P.has_T = ppt2->has_source_T;
-  P.has_E = ppt2->has_source_E;
-  P.has_B = ppt2->has_source_B;
+  P.has_T = st->has_T;
+  P.has_E = st->has_E;
+  P.has_B = st->has_B;
   P.LmaxT = LmaxT;
   P.LmaxP = LmaxP;
-  P.jTT = pbs2->has_J_TT ? pbs2->index_J_TT : -1;
-  P.jEE = pbs2->has_J_EE ? pbs2->index_J_EE : -1;
-  P.jEB = pbs2->has_J_EB ? pbs2->index_J_EB : -1;
+  P.jTT = st->jTT;
+  P.jEE = st->jEE;
+  P.jEB = st->jEB;
   for (int L = 0; L < LOS_MAX_L; ++L)
     for (int im = 0; im < LOS_MAX_MI; ++im) {
       P.srcT[L][im] = P.srcE[L][im] = P.srcB[L][im] = -1;
-      if (L > ppt2->largest_l || im >= ppr2->m_size || ppr2->m[im] > L) continue;
-      if (im > ppr2->index_m_max[L]) continue;
-      const int off = ppt2->lm_array[L][im];
-      if (ppt2->has_source_T && L <= LmaxT) P.srcT[L][im] = ppt2->index_tp2_T + off;
-      if (ppt2->has_source_E && L <= LmaxP) P.srcE[L][im] = ppt2->index_tp2_E + off;
-      if (ppt2->has_source_B && L <= LmaxP) P.srcB[L][im] = ppt2->index_tp2_B + off;
+      if (L > st->largest_l || im >= st->m_size || st->m[im] > L) continue;
+      if (im > st->index_m_max[L]) continue;
+      const int off = st->src_lm[L][im];
+      if (st->has_T && L <= LmaxT) P.srcT[L][im] = st->index_tp2_T + off;
+      if (st->has_E && L <= LmaxP) P.srcE[L][im] = st->index_tp2_E + off;
+      if (st->has_B && L <= LmaxP) P.srcB[L][im] = st->index_tp2_B + off;
     }
   if ((P.has_T && P.jTT < 0) || ((P.has_E || P.has_B) && P.jEE < 0) ||
-      (P.has_B && P.jEB < 0 && ppr2->m_max_song > 0)) {
+      (P.has_B && P.jEB < 0 && st->m_max > 0)) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_load: projection functions missing for a field");
     return _FAILURE_;
   }
@@ -714,21 +693,6 @@ extern "C" int los_engine_load_pairs(los_engine *e, struct precision *ppr,
   HIPCHK(los_stream_set_attrs(tp2), err);
   e->loaded = true;
   return _SUCCESS_;
-}
-
-extern "C" int los_engine_load(los_engine *e, struct precision *ppr, struct precision2 *ppr2,
-                               struct perturbs2 *ppt2, struct bessels *pbs,
-                               struct bessels2 *pbs2, struct transfers2 *ptr2, int k1_begin,
-                               int k1_end, char *err)
-{
-  std::vector<int> k1, k2;
-  for (int i1 = k1_begin; i1 < k1_end; ++i1)
-    for (int i2 = 0; i2 <= i1; ++i2) {
-      k1.push_back(i1);
-      k2.push_back(i2);
-    }
-  return los_engine_load_pairs(e, ppr, ppr2, ppt2, pbs, pbs2, ptr2, k1.data(), k2.data(),
-                               (int)k1.size(), err);
 }
 
 // Re-layout and k3 spline of the loaded sources (enqueued, not waited for).
@@ -1230,7 +1194,10 @@ extern "C" int los_engine_last_kernel_ms(los_engine *e, float *ms_los, float *ms
   return _SUCCESS_;
 }
 
-extern "C" int los_engine_store(los_engine *e, struct transfers2 *ptr2, char *err)
+// Results into SONG's transfer arrays, with the reference's NaN check
+// (source/transfer2.c:2332-2337, on in its default DEBUG build): a NaN
+// transfer fails the store with the reference's message.
+extern "C" int los_engine_store_stage(los_engine *e, const los_stage *st, char *err)
 {
   if (!e->loaded) {
     snprintf(err, _ERRORMSGSIZE_, "los_engine_store: nothing loaded");
@@ -1244,13 +1211,20 @@ extern "C" int los_engine_store(los_engine *e, struct transfers2 *ptr2, char *er
   HIPCHK(hipStreamSynchronize(e->stream), err);
   for (int tt = 0; tt < e->tt2_size; ++tt)
     for (const los_pair &p : e->pairs) {
-      if (!ptr2->has_allocated_transfers[p.k1]) {
+      if (!st->has_transfers[p.k1]) {
         snprintf(err, _ERRORMSGSIZE_, "los_engine_store: k1 level %d not allocated", p.k1);
         return _FAILURE_;
       }
-      memcpy(ptr2->transfer[tt][p.k1][p.k2], h.data() + (size_t)tt * e->F + p.f_off,
-             p.nf * sizeof(double));
-      ptr2->count_memorised_transfers += p.nf;
+      const double *v = h.data() + (size_t)tt * e->F + p.f_off;
+      for (int k = 0; k < p.nf; ++k)
+        if (std::isnan(v[k])) {
+          snprintf(err, _ERRORMSGSIZE_,
+                   "found nan in second-order transfer function (index_tt=%d, index_k1=%d, "
+                   "index_k2=%d, index_k3=%d)", tt, p.k1, p.k2, k);
+          return _FAILURE_;
+        }
+      memcpy(st->transfer[tt][p.k1][p.k2], v, p.nf * sizeof(double));
+      *st->count_memorised += p.nf;
     }
   return _SUCCESS_;
 }

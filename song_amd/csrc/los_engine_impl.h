@@ -10,7 +10,7 @@
 #include <vector>
 
 #include "los_device.h"
-#include "los_internal.h"
+#include "los_common.h"
 
 extern "C" hipError_t los_launch_relayout_spline(const los_pair *pairs, int npairs, int nc_max,
                                                  const double *kc, const double *spl,

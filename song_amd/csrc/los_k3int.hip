@@ -28,7 +28,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "los_internal.h"
+#include "los_common.h"
 
 #define KCHK(call)                                                                           \
   do {                                                                                       \

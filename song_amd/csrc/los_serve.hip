@@ -274,11 +274,21 @@ static void scheduler(los_server *S)
   }
 }
 
-extern "C" int los_server_create(int device, int form, double hbm_budget_bytes, int n_slots,
-                                 struct precision *ppr, struct precision2 *ppr2,
-                                 struct perturbs2 *ppt2, struct bessels *pbs,
-                                 struct bessels2 *pbs2, struct transfers2 *ptr2,
-                                 los_server **out, char *err)
+// every (k1,k2) pair of the run
+static int load_all(los_engine *e, const los_stage *st, char *err)
+{
+  std::vector<int> k1, k2;
+  for (int i1 = 0; i1 < st->k_size; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2) {
+      k1.push_back(i1);
+      k2.push_back(i2);
+    }
+  return los_engine_load_stage(e, st, k1.data(), k2.data(), (int)k1.size(), err);
+}
+
+extern "C" int los_server_create_stage(int device, int form, double hbm_budget_bytes,
+                                       int n_slots, const los_stage *st, los_server **out,
+                                       char *err)
 {
   if (n_slots < 1 || n_slots > 8) {
     snprintf(err, _ERRORMSGSIZE_, "los_server_create: %d slots (1 to 8)", n_slots);
@@ -291,7 +301,7 @@ extern "C" int los_server_create(int device, int form, double hbm_budget_bytes, 
   los_server *S = new los_server();
   S->budget = hbm_budget_bytes;
   if (los_engine_create(device, &S->e, err) || los_engine_set_form(S->e, form, err) ||
-      los_engine_load(S->e, ppr, ppr2, ppt2, pbs, pbs2, ptr2, 0, ppt2->k_size, err)) {
+      load_all(S->e, st, err)) {
     if (S->e) los_engine_destroy(S->e);
     delete S;
     return _FAILURE_;
@@ -325,9 +335,9 @@ extern "C" int los_server_create(int device, int form, double hbm_budget_bytes, 
   for (size_t q = 0; q < e->pairs.size(); ++q)
     S->pair_of[std::make_pair(e->pairs[q].k1, e->pairs[q].k2)] = (int)q;
   S->lm_l.assign(e->n_lm, 0);
-  for (int il = 0; il < ptr2->l_size; ++il)
-    for (int im = 0; im <= ppr2->index_m_max[ptr2->l[il]]; ++im)
-      S->lm_l[ptr2->lm_array[il][im]] = ptr2->l[il];
+  for (int il = 0; il < st->l_size; ++il)
+    for (int im = 0; im <= st->index_m_max[st->l[il]]; ++im)
+      S->lm_l[st->lm_array[il][im]] = st->l[il];
   const std::vector<double> &pc =
       S->shape.kind == 1 ? e->pos_cost_sweep : e->pos_cost;
   S->fcost.assign(e->F, 0.0);

@@ -662,6 +662,185 @@ int transfer2_integrate(struct precision *ppr, struct precision2 *ppr2, struct p
 }
 
 /* ---------------------------------------------------------------------------
+ * The flat descriptor of the GPU side (los_stage.h). This unit is the only
+ * one that reads SONG's structures; recompiled against SONG's headers, it
+ * adapts the whole library to their layout.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  struct precision *ppr;
+  struct precision2 *ppr2;
+  struct perturbs2 *ppt2;
+  struct bessels *pbs;
+  struct bessels2 *pbs2;
+  struct transfers2 *ptr2;
+} song_structs;
+
+static int stage_k3_list(const los_stage *st, int i1, int i2, double *out, char *err)
+{
+  const song_structs *q = (const song_structs *)st->song;
+  int last_used = 0;
+  if (transfer2_get_k3_list(q->ppr, q->ppr2, q->ppt2, q->pbs, q->pbs2, q->ptr2, i1, i2, out,
+                            &last_used) == _FAILURE_) {
+    memcpy(err, q->ptr2->error_message, _ERRORMSGSIZE_);
+    return _FAILURE_;
+  }
+  return _SUCCESS_;
+}
+
+static void stage_fill(los_stage *st, song_structs *q)
+{
+  struct precision *ppr = q->ppr;
+  struct precision2 *ppr2 = q->ppr2;
+  struct perturbs2 *ppt2 = q->ppt2;
+  struct bessels2 *pbs2 = q->pbs2;
+  struct transfers2 *ptr2 = q->ptr2;
+  memset(st, 0, sizeof *st);
+  st->tau_sources = ptr2->tau_sampling == sources_tau_sampling;
+  st->linear_J = ppr->bessels_interpolation == linear_interpolation;
+  st->cubic_k3 = ppr2->sources_k3_interpolation == cubic_interpolation;
+  st->tau_size = ppt2->tau_size;
+  st->tau = ppt2->tau_sampling;
+  st->tau0 = ptr2->tau0;
+  st->has_T = ppt2->has_source_T;
+  st->has_E = ppt2->has_source_E;
+  st->has_B = ppt2->has_source_B;
+  st->LmaxT = ppr2->l_max_los_t;
+  st->LmaxP = ppr2->l_max_los_p;
+  st->m_max = ppr2->m_max_song;
+  st->largest_l = ppt2->largest_l;
+  st->m_size = ppr2->m_size;
+  st->m = ppr2->m;
+  st->index_m_max = ppr2->index_m_max;
+  st->l_size = ptr2->l_size;
+  st->l = ptr2->l;
+  st->lm_array = ptr2->lm_array;
+  st->src_lm = ppt2->lm_array;
+  st->tt2_size = ptr2->tt2_size;
+  st->n_transfers = ptr2->n_transfers;
+  st->index_tt2_T = ptr2->index_tt2_T;
+  st->index_tt2_E = ptr2->index_tt2_E;
+  st->index_tt2_B = ptr2->index_tt2_B;
+  st->tp2_size = ppt2->tp2_size;
+  st->index_tp2_T = ppt2->index_tp2_T;
+  st->index_tp2_E = ppt2->index_tp2_E;
+  st->index_tp2_B = ppt2->index_tp2_B;
+  st->k_size = ppt2->k_size;
+  st->sources = ppt2->sources;
+  st->has_sources = ppt2->has_allocated_sources;
+  st->k3 = ppt2->k3;
+  st->k3_size = ppt2->k3_size;
+  st->k3_size_max = ptr2->k3_size_max;
+  st->k_size_k1k2 = ptr2->k_size_k1k2;
+  st->k_phys_start = ptr2->k_physical_start_k1k2;
+  st->k_phys_size = ptr2->k_physical_size_k1k2;
+  st->k_max_k1k2 = ptr2->k_max_k1k2;
+  st->k3_list = stage_k3_list;
+  st->song = q;
+  st->J_id = pbs2;
+  st->J_size = pbs2->J_size;
+  st->L_size = pbs2->L_size;
+  st->xx_size = pbs2->xx_size;
+  st->xx_step = pbs2->xx_step;
+  st->xx_max = pbs2->xx_max;
+  st->xx = pbs2->xx;
+  st->jTT = pbs2->has_J_TT ? pbs2->index_J_TT : -1;
+  st->jEE = pbs2->has_J_EE ? pbs2->index_J_EE : -1;
+  st->jEB = pbs2->has_J_EB ? pbs2->index_J_EB : -1;
+  st->xmin_J = pbs2->index_xmin_J;
+  st->x_size_J = pbs2->x_size_J;
+  st->J = pbs2->J_Llm_x;
+  st->transfer = ptr2->transfer;
+  st->has_transfers = ptr2->has_allocated_transfers;
+  st->count_memorised = &ptr2->count_memorised_transfers;
+}
+
+/* The engine and server entry points of include/song_los.h that take SONG's
+   structures: the descriptor, then the engine's own code. */
+int los_engine_load_pairs(los_engine *eng, struct precision *ppr, struct precision2 *ppr2,
+                          struct perturbs2 *ppt2, struct bessels *pbs, struct bessels2 *pbs2,
+                          struct transfers2 *ptr2, const int *index_k1, const int *index_k2,
+                          int npairs, char *errmsg)
+{
+  song_structs q = {ppr, ppr2, ppt2, pbs, pbs2, ptr2};
+  los_stage st;
+  stage_fill(&st, &q);
+  return los_engine_load_stage(eng, &st, index_k1, index_k2, npairs, errmsg);
+}
+
+int los_engine_load(los_engine *eng, struct precision *ppr, struct precision2 *ppr2,
+                    struct perturbs2 *ppt2, struct bessels *pbs, struct bessels2 *pbs2,
+                    struct transfers2 *ptr2, int k1_begin, int k1_end, char *errmsg)
+{
+  const int n = k1_end > k1_begin ? (k1_end * (k1_end + 1) - k1_begin * (k1_begin + 1)) / 2 : 0;
+  int *k1 = NULL, *k2 = NULL;
+  los_alloc(k1, (size_t)(n > 0 ? n : 1) * sizeof(int), errmsg);
+  k2 = malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  if (k2 == NULL) {
+    free(k1);
+    los_fail(errmsg, "los_engine_load: out of memory");
+  }
+  int c = 0;
+  for (int i1 = k1_begin; i1 < k1_end; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2) {
+      k1[c] = i1;
+      k2[c++] = i2;
+    }
+  const int rc = los_engine_load_pairs(eng, ppr, ppr2, ppt2, pbs, pbs2, ptr2, k1, k2, c, errmsg);
+  free(k1);
+  free(k2);
+  return rc;
+}
+
+int los_engine_store(los_engine *eng, struct transfers2 *ptr2, char *errmsg)
+{
+  los_stage st;
+  memset(&st, 0, sizeof st);
+  st.transfer = ptr2->transfer;
+  st.has_transfers = ptr2->has_allocated_transfers;
+  st.count_memorised = &ptr2->count_memorised_transfers;
+  return los_engine_store_stage(eng, &st, errmsg);
+}
+
+int los_server_create(int device, int form, double hbm_budget_bytes, int n_slots,
+                      struct precision *ppr, struct precision2 *ppr2, struct perturbs2 *ppt2,
+                      struct bessels *pbs, struct bessels2 *pbs2, struct transfers2 *ptr2,
+                      los_server **out, char *errmsg)
+{
+  song_structs q = {ppr, ppr2, ppt2, pbs, pbs2, ptr2};
+  los_stage st;
+  stage_fill(&st, &q);
+  return los_server_create_stage(device, form, hbm_budget_bytes, n_slots, &st, out, errmsg);
+}
+
+/* Device bytes of one k1 batch's pairs beyond the resident J tables: the
+   sources three times (as SONG lays them out, re-laid out, spline second
+   derivatives), and per fine k3 the outputs, the per-(k3, tau) tables of the
+   sweep and k-lane kernels (the autotuner may build both) and small lists.
+   SONG_LOS_BATCH_BYTES in the environment replaces the budget (bytes), so that
+   smaller batches can be asked for. */
+static double batch_bytes(const los_stage *st, int k1_begin, int k1_end)
+{
+  const int NL = (st->LmaxT > st->LmaxP ? st->LmaxT : st->LmaxP) + 1;
+  double b = 0;
+  for (int i1 = k1_begin; i1 < k1_end; ++i1)
+    for (int i2 = 0; i2 <= i1; ++i2) {
+      const double nf = st->k_size_k1k2[i1][i2];
+      b += 3.0 * 8.0 * st->tp2_size * (double)st->tau_size * st->k3_size[i1][i2];
+      b += nf * (8.0 * st->tt2_size + 64.0);
+      b += nf * st->tau_size * (16.0 + 8.0 * 3 * st->m_size * NL);  /* sweep tables */
+      b += nf * st->tau_size * (16.0 + 8.0 * st->tp2_size);         /* k-lane tables */
+    }
+  return b;
+}
+
+static double batch_budget(los_engine *eng)
+{
+  const char *v = getenv("SONG_LOS_BATCH_BYTES");
+  if (v != NULL && atof(v) > 0) return atof(v);
+  return los_engine_batch_budget(eng);
+}
+
+/* ---------------------------------------------------------------------------
  * transfer2_init / transfer2_free. source/transfer2.c:79-621, :769-838
  * ------------------------------------------------------------------------- */
 int transfer2_init(struct precision *ppr, struct precision2 *ppr2, struct background *pba,
@@ -699,17 +878,19 @@ int transfer2_init(struct precision *ppr, struct precision2 *ppr2, struct backgr
     return _FAILURE_;
   }
 
-  /* k1 batches sized by device memory: sources + spline table + outputs */
+  /* k1 batches sized by device memory */
+  song_structs q = {ppr, ppr2, ppt2, pbs, pbs2, ptr2};
+  los_stage st;
+  stage_fill(&st, &q);
   const int nk = ppt2->k_size;
   int k1_begin = 0;
   int status = _SUCCESS_;
   while (k1_begin < nk && status == _SUCCESS_) {
     int k1_end = k1_begin + 1;
-    double budget = los_engine_batch_budget(eng);
-    double bytes = los_engine_batch_bytes(ppr2, ppt2, ptr2, k1_begin, k1_end);
-    while (k1_end < nk &&
-           bytes + los_engine_batch_bytes(ppr2, ppt2, ptr2, k1_end, k1_end + 1) < budget) {
-      bytes += los_engine_batch_bytes(ppr2, ppt2, ptr2, k1_end, k1_end + 1);
+    double budget = batch_budget(eng);
+    double bytes = batch_bytes(&st, k1_begin, k1_end);
+    while (k1_end < nk && bytes + batch_bytes(&st, k1_end, k1_end + 1) < budget) {
+      bytes += batch_bytes(&st, k1_end, k1_end + 1);
       k1_end++;
     }
     for (int i1 = k1_begin; i1 < k1_end && status == _SUCCESS_; ++i1) {
@@ -729,7 +910,7 @@ int transfer2_init(struct precision *ppr, struct precision2 *ppr2, struct backgr
                                ptr2->error_message);
     float ms = 0;
     if (status == _SUCCESS_) status = los_engine_run(eng, &ms, ptr2->error_message);
-    if (status == _SUCCESS_) status = los_engine_store(eng, ptr2, ptr2->error_message);
+    if (status == _SUCCESS_) status = los_engine_store_stage(eng, &st, ptr2->error_message);
     for (int i1 = k1_begin; i1 < k1_end && status == _SUCCESS_; ++i1) {
       if (perturb2_free_k1_level(ppt2, i1) == _FAILURE_) {
         memcpy(ptr2->error_message, ppt2->error_message, _ERRORMSGSIZE_);

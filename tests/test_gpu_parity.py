@@ -151,3 +151,50 @@ def test_fp64_bit_exact_sources_from_disk(oracle, tmp_path):
     got = s.gather()
     assert np.array_equal(got, ref)
     s.close()
+
+
+@pytest.mark.gpu
+def test_store_transfers_files_equal_oracle(oracle, tmp_path):
+    """transfer2_init in store_transfers_to_disk mode (source/transfer2.c:549-567,
+    :2929-2963): each k1 level is written to SONG's transfers_%03d.dat files and freed;
+    the files equal, byte for byte, those the oracle writes from the same inputs."""
+    import os
+    cfg = L.Config(**CASES["TEB_m012"])
+    print("case store_transfers", flush=True)
+    dirs = {}
+    for who in ("oracle", "gpu"):
+        d = tmp_path / who
+        d.mkdir()
+        s = L.Scenario(cfg)
+        s.set("store_transfers_to_disk", 1)
+        s.set_string("transfers_dir", str(d))
+        if who == "oracle":
+            assert oracle.los_ref_transfer2_init(*s.structs()) == 0, s.error_message()
+            s._inited = True
+            s.free_transfers(lambda a, b, c: oracle.los_ref_transfer2_free(a, b, c))
+        else:
+            s.transfer2_init()
+        s.close()
+        dirs[who] = d
+    names = sorted(os.listdir(dirs["oracle"]))
+    assert names and names == sorted(os.listdir(dirs["gpu"]))
+    for n in names:
+        a = (dirs["oracle"] / n).read_bytes()
+        b = (dirs["gpu"] / n).read_bytes()
+        assert len(a) > 0 and a == b, n
+
+
+@pytest.mark.gpu
+def test_small_batches_bit_exact(oracle, monkeypatch):
+    """A device budget that holds one k1 level at a time (SONG_LOS_BATCH_BYTES): the k1
+    batches of transfer2_init run one after another and the result still equals the
+    oracle's bit for bit."""
+    cfg = L.Config(**CASES["TEB_m012"])
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    monkeypatch.setenv("SONG_LOS_BATCH_BYTES", "1")
+    print("case small_batches", flush=True)
+    s = L.Scenario(cfg)
+    s.transfer2_init()
+    got = s.gather()
+    s.close()
+    assert np.array_equal(got, ref)

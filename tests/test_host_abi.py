@@ -238,3 +238,32 @@ def test_transfers2_layout_is_songs():
     want = [getattr(Song, f).offset for f, _ in fields] + [C.sizeof(Song)]
     assert list(buf[:n]) == want
     assert LBL == 64
+
+
+def test_hip_sources_never_see_song_structs():
+    """Only host C units read SONG's structures (los_transfer2.c fills the flat
+    descriptor of los_stage.h for the GPU side): no HIP source, and no header a HIP
+    source includes, includes the struct definitions, so recompiling the host unit
+    against SONG's headers adapts the library to SONG's layouts."""
+    import re
+    csrc = os.path.join(os.path.dirname(L.__file__), "csrc")
+    inc = re.compile(r'#include\s+"([^"]+)"')
+    banned = {"song_los_types.h", "transfer2.h", "los_internal.h"}
+
+    def closure(name, seen):
+        if name in seen:
+            return
+        seen.add(name)
+        path = os.path.join(csrc, name)
+        if not os.path.exists(path):
+            return
+        for dep in inc.findall(open(path).read()):
+            base = os.path.basename(dep)
+            if base in banned:
+                raise AssertionError(f"{name} includes {dep}")
+            closure(base, seen)
+
+    hips = [f for f in os.listdir(csrc) if f.endswith(".hip")]
+    assert len(hips) >= 6
+    for h in hips:
+        closure(h, set())
