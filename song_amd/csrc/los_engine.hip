@@ -724,7 +724,7 @@ static int form_shapes(const los_engine *e, int form, los_shape *out, int cap)
   int n = 0;
   if (form == LOS_FORM_FP64) {
     all[n++] = los_shape{0, 1, 0, 0};
-    for (int nk = 1; nk <= 4; nk *= 2)
+    for (int nk = 1; nk <= 8; nk *= 2)
       for (int p = 0; p < 2; ++p) all[n++] = los_shape{1, nk, 0, p};
     for (int b = 1; b <= 4; b *= 2) all[n++] = los_shape{2, 64, b, 0};
   } else {

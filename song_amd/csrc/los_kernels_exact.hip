@@ -409,6 +409,7 @@ hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_c
     case 1: return launch_sweep_exact<1>(P, pers != 0, n_cu, s);
     case 2: return launch_sweep_exact<2>(P, pers != 0, n_cu, s);
     case 4: return launch_sweep_exact<4>(P, pers != 0, n_cu, s);
+    case 8: return launch_sweep_exact<8>(P, pers != 0, n_cu, s);
     default: return hipErrorInvalidValue;
   }
 }

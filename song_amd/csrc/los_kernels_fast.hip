@@ -112,6 +112,9 @@ hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int
     case 4:
       return fp32 ? launch_sweep<StreamFP32C, 4>(P, p, q, n_cu, s)
                   : launch_sweep<StreamFP64, 4>(P, p, q, n_cu, s);
+    case 8:
+      return fp32 ? launch_sweep<StreamFP32C, 8>(P, p, q, n_cu, s)
+                  : launch_sweep<StreamFP64, 8>(P, p, q, n_cu, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -158,7 +161,7 @@ int los_shapes(int fp32, int tp2_size, los_shape *out, int cap)
   LOS_SHAPES(LOS_LIST)
 #undef LOS_LIST
   // sweep: ch = 1 reads the FP32 pair table (FP64 always reads its pair table)
-  for (int nk = 1; nk <= 4; nk *= 2)
+  for (int nk = 1; nk <= 8; nk *= 2)
     for (int c = 0; c <= fp32; ++c)
       for (int p = 0; p < 2 && n < cap; ++p) out[n++] = los_shape{1, nk, c, p};
   for (int b = 1; b <= 4 && n < cap; b *= 2) out[n++] = los_shape{2, 64, b, 0};

@@ -17,11 +17,12 @@ from tests.test_gpu_parity import CASES
 _STREAM = [(0, nk, ch, p) for nk in (2, 4, 8) for ch in (16, 32) for p in (0, 1)]
 _KLANE = [(2, 64, b, 0) for b in (1, 2, 4)]
 # sweep: ch = 1 is the FP32 pair table (the FP64 forms always read their pair table)
-SHAPES = {L.FORM_FP64_FAST: _STREAM + [(1, nk, 0, p) for nk in (1, 2, 4) for p in (0, 1)] + _KLANE,
-          L.FORM_FP32C: _STREAM + [(1, nk, c, p) for nk in (1, 2, 4) for c in (0, 1)
+SHAPES = {L.FORM_FP64_FAST: _STREAM + [(1, nk, 0, p) for nk in (1, 2, 4, 8) for p in (0, 1)] +
+          _KLANE,
+          L.FORM_FP32C: _STREAM + [(1, nk, c, p) for nk in (1, 2, 4, 8) for c in (0, 1)
                                    for p in (0, 1)] + _KLANE}
 # the plain form's kernels: per-k3 workgroups, sweep (dispatch and persistent), k-lane
-FP64_SHAPES = ([(0, 1, 0, 0)] + [(1, nk, 0, p) for nk in (1, 2, 4) for p in (0, 1)] +
+FP64_SHAPES = ([(0, 1, 0, 0)] + [(1, nk, 0, p) for nk in (1, 2, 4, 8) for p in (0, 1)] +
                [(2, 64, b, 0) for b in (1, 2, 4)])
 
 # log k in [1e-5, 0.2], 8 coarse k3 per pair, fine step 5e-4: fine k3 counts from 8
