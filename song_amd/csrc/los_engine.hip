@@ -696,8 +696,18 @@ extern "C" int los_engine_load_stage(los_engine *e, const los_stage *st, const i
 }
 
 // Re-layout and k3 spline of the loaded sources (enqueued, not waited for).
-int enqueue_prep(los_engine *e, char *err)
+// Whether a sweep shape takes its tables straight from the raw sources.
+static bool use_fused(const los_engine *e, const los_shape *sh)
 {
+  return sh != nullptr && sh->kind == 1 && e->fused_prep &&
+         los_sweep_fused_fits(e->P.tp2_size, e->nc_max);
+}
+
+// Source re-layout and k3 spline for the kernels that read them (every shape
+// but a sweep shape with fused tables; sh == nullptr: always).
+int enqueue_prep(los_engine *e, const los_shape *sh, char *err)
+{
+  if (use_fused(e, sh)) return _SUCCESS_;
   const int tp2 = e->P.tp2_size;
   const int np = (int)e->pairs.size();
   HIPCHK(los_launch_relayout_spline(e->dpairs, np, e->nc_max, e->dkc, e->dspl, e->draw, e->dsrc,
@@ -789,17 +799,28 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
 // Per-(k3, tau) tables of a sweep or k-lane shape for the loaded batch
 // (enqueued on the engine stream).
 static int enqueue_tables(los_engine *e, const los_shape *sh, char *err, int f_begin = 0,
-                          int f_end = -1, hipStream_t s = nullptr)
+                          int f_end = -1, hipStream_t s = nullptr, int p_begin = 0,
+                          int p_end = -1)
 {
   const int fp32 = e->form == LOS_FORM_FP32C;
   if (f_end < 0) f_end = e->F;
+  if (p_end < 0) p_end = (int)e->pairs.size();
   if (!s) s = e->stream;
   if (sh->kind == 1) {
     if (e->sweep_form != e->form) {
       snprintf(err, _ERRORMSGSIZE_, "enqueue_tables: sweep tables not allocated for this form");
       return _FAILURE_;
     }
-    if (e->form == LOS_FORM_FP64)
+    if (use_fused(e, sh)) {
+      if (e->form == LOS_FORM_FP64)
+        HIPCHK(los_launch_sweep_prep_fused_exact(&e->P, e->draw, e->dspl, e->nc_max, p_begin,
+                                                 p_end, s),
+               err);
+      else
+        HIPCHK(los_launch_sweep_prep_fused(fp32, &e->P, e->draw, e->dspl, e->nc_max, p_begin,
+                                           p_end, s),
+               err);
+    } else if (e->form == LOS_FORM_FP64)
       HIPCHK(los_launch_sweep_prep_exact(&e->P, f_begin, f_end, s), err);
     else
       HIPCHK(los_launch_sweep_prep(fp32, &e->P, f_begin, f_end, s), err);
@@ -851,10 +872,12 @@ static int enqueue_pipelined(los_engine *e, const los_shape *sh, char *err)
   HIPCHK(hipStreamWaitEvent(e->stream2, e->ev_fork, 0), err);
   for (int c = 0; c < e->n_chunk; ++c) {
     const int p0 = e->chunk_pair[c], p1 = e->chunk_pair[c + 1];
-    HIPCHK(los_launch_relayout_spline(e->dpairs + p0, p1 - p0, e->nc_max, e->dkc, e->dspl, e->draw,
-                                      e->dsrc, e->ddd, e->ntau, tp2, e->P.cubic_k3, e->stream2),
-           err);
-    if (enqueue_tables(e, sh, err, e->chunk_f[c], e->chunk_f[c + 1], e->stream2))
+    if (!use_fused(e, sh))
+      HIPCHK(los_launch_relayout_spline(e->dpairs + p0, p1 - p0, e->nc_max, e->dkc, e->dspl,
+                                        e->draw, e->dsrc, e->ddd, e->ntau, tp2, e->P.cubic_k3,
+                                        e->stream2),
+             err);
+    if (enqueue_tables(e, sh, err, e->chunk_f[c], e->chunk_f[c + 1], e->stream2, p0, p1))
       return _FAILURE_;
     HIPCHK(hipEventRecord(e->ev_chunk[c], e->stream2), err);
     HIPCHK(hipStreamWaitEvent(e->stream, e->ev_chunk[c], 0), err);
@@ -1007,7 +1030,7 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
     }
   }
   if (reps < 1) reps = 1;
-  if (ensure_j32(e, err) || enqueue_prep(e, err)) return _FAILURE_;
+  if (ensure_j32(e, err)) return _FAILURE_;
   los_shape all[48], cand[48];
   const int na = form_shapes(e, form, all, 48);
   int nc = 0;
@@ -1021,11 +1044,13 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
   float best = 1e30f;
   int ibest = 0;
   for (int i = 0; i < nc; ++i) {
+    // each shape timed on whole steps: its own preparation and tables, then its kernel
     if (ensure_tables(e, &cand[i], err)) return _FAILURE_;
-    if (enqueue_los(e, &cand[i], err)) return _FAILURE_;  // warm-up
+    if (enqueue_prep(e, &cand[i], err) || enqueue_los(e, &cand[i], err))  // warm-up
+      return _FAILURE_;
     HIPCHK(hipEventRecord(e->ev[1], e->stream), err);
     for (int r = 0; r < reps; ++r)
-      if (enqueue_los(e, &cand[i], err)) return _FAILURE_;
+      if (enqueue_prep(e, &cand[i], err) || enqueue_los(e, &cand[i], err)) return _FAILURE_;
     HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
     HIPCHK(hipEventSynchronize(e->ev[2]), err);
     float ms = 0;
@@ -1106,7 +1131,7 @@ extern "C" int los_engine_capture(los_engine *e, char *err)
   if (pipelined(e, sh) && e->pipeline) {
     rc = enqueue_pipelined(e, sh, err);
   } else {
-    rc = enqueue_prep(e, err);
+    rc = enqueue_prep(e, sh, err);
     if (rc == _SUCCESS_) rc = enqueue_los(e, sh, err);
   }
   hipGraph_t g = nullptr;
@@ -1165,13 +1190,21 @@ extern "C" int los_engine_run(los_engine *e, float *ms, char *err)
   // ev0 .. ev1: source re-layout, k3 spline and the shape's per-(k3, tau)
   // tables; ev1 .. ev2: the LOS kernel alone
   HIPCHK(hipEventRecord(e->ev[0], e->stream), err);
-  if (enqueue_prep(e, err)) return _FAILURE_;
+  if (enqueue_prep(e, sh, err)) return _FAILURE_;
   if (enqueue_los(e, sh, err, e->ev[1])) return _FAILURE_;
   HIPCHK(hipEventRecord(e->ev[2], e->stream), err);
   HIPCHK(hipEventSynchronize(e->ev[2]), err);
   HIPCHK(hipEventElapsedTime(&e->ms_spline, e->ev[0], e->ev[1]), err);
   HIPCHK(hipEventElapsedTime(&e->ms_los, e->ev[1], e->ev[2]), err);
   if (ms) *ms = e->ms_spline + e->ms_los;
+  return _SUCCESS_;
+}
+
+extern "C" int los_engine_set_fused_prep(los_engine *e, int on, char *err)
+{
+  (void)err;
+  if ((on != 0) != e->fused_prep) drop_graph(e);
+  e->fused_prep = on != 0;
   return _SUCCESS_;
 }
 

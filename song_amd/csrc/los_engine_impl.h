@@ -29,6 +29,13 @@ extern "C" hipError_t los_launch_sweep_prep_exact(const los_params *P, int f_beg
                                                   hipStream_t s);
 extern "C" hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_cu,
                                              hipStream_t s);
+extern "C" hipError_t los_launch_sweep_prep_fused_exact(const los_params *P, const double *raw,
+                                                        const double *spl, int nc_max,
+                                                        int p_begin, int p_end, hipStream_t s);
+extern "C" hipError_t los_launch_sweep_prep_fused(int fp32, const los_params *P, const double *raw,
+                                                  const double *spl, int nc_max, int p_begin,
+                                                  int p_end, hipStream_t s);
+extern "C" int los_sweep_fused_fits(int tp2, int nc_max);
 extern "C" hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst,
                                              const int *xminp, long rows, int X, int NLP,
                                              hipStream_t s);
@@ -105,6 +112,9 @@ struct los_engine {
   int tt2_size = 0;
   los_params P;
   bool loaded = false;
+  // sweep tables straight from the raw sources (los_sweep_prep_fused) when the
+  // batch fits its LDS budget and fused_prep is on (the default)
+  bool fused_prep = true;
   bool pipeline = false;  // pipelined steps (chunks_forced >= 2 and a sweep shape)
 
   double *staging = nullptr;  // pinned
@@ -170,7 +180,7 @@ static inline int upload_vec(los_engine *e, T *&dptr, const std::vector<T> &v, c
 void split_xcd(const std::vector<los_tile> &tiles, const std::vector<double> &pos_cost, int out[9]);
 int nk_slot(int nk);
 std::vector<los_tile> make_tiles(const std::vector<int> &job_end, int nk);
-int enqueue_prep(los_engine *e, char *err);
+int enqueue_prep(los_engine *e, const los_shape *sh, char *err);
 int ensure_j32(los_engine *e, char *err);
 int current_shape(los_engine *e, const los_shape **sh, char *err);
 bool sweep_fits(const los_engine *e, int form, const los_shape &sh);

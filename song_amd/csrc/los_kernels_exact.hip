@@ -403,6 +403,15 @@ hipError_t los_launch_sweep_prep_exact(const los_params *P, int f_begin, int f_e
   return launch_sweep_prep<double, false>(P, f_begin, f_end, s);
 }
 
+hipError_t los_launch_sweep_prep_fused_exact(const los_params *P, const double *raw,
+                                             const double *spl, int nc_max, int p_begin,
+                                             int p_end, hipStream_t s)
+{
+  return launch_sweep_prep_fused<double, false>(P, raw, spl, nc_max, p_begin, p_end, s);
+}
+
+int los_sweep_fused_fits(int tp2, int nc_max) { return sweep_fused_fits(tp2, nc_max) ? 1 : 0; }
+
 hipError_t los_launch_sweep_exact(const los_params *P, int nk, int pers, int n_cu, hipStream_t s)
 {
   switch (nk) {

@@ -98,6 +98,14 @@ hipError_t los_launch_sweep_pairs(int fp32, const void *src, void *dst, const in
   return hipGetLastError();
 }
 
+hipError_t los_launch_sweep_prep_fused(int fp32, const los_params *P, const double *raw,
+                                       const double *spl, int nc_max, int p_begin, int p_end,
+                                       hipStream_t s)
+{
+  return fp32 ? launch_sweep_prep_fused<float, true>(P, raw, spl, nc_max, p_begin, p_end, s)
+              : launch_sweep_prep_fused<double, true>(P, raw, spl, nc_max, p_begin, p_end, s);
+}
+
 hipError_t los_launch_sweep(int fp32, const los_params *P, int nk, int pers, int pairs, int n_cu,
                             hipStream_t s)
 {

@@ -310,7 +310,7 @@ extern "C" int los_server_create_stage(int device, int form, double hbm_budget_b
   // resident preparation: source re-layout and k3 spline, kernel shape, and the
   // per-(k3, tau) tables of a sweep shape
   const los_shape *sh = nullptr;
-  int rc = enqueue_prep(e, err);
+  int rc = enqueue_prep(e, nullptr, err);
   if (rc == _SUCCESS_ && hipStreamSynchronize(e->stream) != hipSuccess) {
     snprintf(err, _ERRORMSGSIZE_, "los_server_create: source preparation failed");
     rc = _FAILURE_;

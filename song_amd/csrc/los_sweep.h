@@ -265,6 +265,172 @@ los_sweep_prep(los_params P, int ntb, int f_begin, int f_end)
 }
 
 // ---------------------------------------------------------------------------
+// The same tables straight from the sources as SONG lays them out, with the k3
+// spline in the same kernel (no re-laid sources or second derivatives in HBM).
+// A workgroup takes one pair and TB consecutive time steps: it stages the
+// pair's [tp][tau][kc] rows of those steps in LDS (each (tp) block of TB rows
+// is one contiguous run), solves the spline of every (tp, tau) column in LDS
+// with the operations of los_relayout_spline (so the second derivatives are
+// the same values), and writes the records ixa[f][t] and sf[f][t][mi][field][L]
+// of every fine k3 of the pair with the expressions of los_sweep_prep.
+// LDS: values and second derivatives, [tp * TB + t][ls] each, ls odd.
+// ---------------------------------------------------------------------------
+#define LOS_SPF_THREADS 256
+
+template <class R, bool WITH_DT, int NL>
+__global__ void __launch_bounds__(LOS_SPF_THREADS)
+los_sweep_prep_fused(los_params P, const double *__restrict__ raw, const double *__restrict__ spl,
+                     int p_begin, int nchunk, int TB, int ls)
+{
+  HIP_DYNAMIC_SHARED(double, lds)
+  const int p = p_begin + (int)(blockIdx.x / nchunk);
+  const int t0 = (int)(blockIdx.x % nchunk) * TB;
+  const los_pair pr = P.pairs[p];
+  const int n = pr.nc, tp2 = P.tp2_size;
+  const long ntau = P.ntau;
+  const int nt = min(TB, (int)(ntau - t0));
+  double *yv = lds;                         // [tp * TB + t][ls] source values
+  double *dv = lds + (long)tp2 * TB * ls;   // the same, second derivatives
+  const bool cubic = P.cubic_k3 != 0;
+
+  // each (tp) block of nt rows [tau][kc] is one contiguous run of nt * n values
+  const int run = nt * n;
+  for (int q = threadIdx.x; q < tp2 * run; q += LOS_SPF_THREADS) {
+    const int tp = q / run, r = q - tp * run;
+    const int t = r / n, kc = r - t * n;
+    const double v = raw[pr.src_off + (long)tp * ntau * n + (long)t0 * n + r];
+    yv[(tp * TB + t) * ls + kc] = v;
+    dv[(tp * TB + t) * ls + kc] = v;
+  }
+  __syncthreads();
+  if (cubic) {
+    const double *x = P.kc + pr.kc_off;
+    const double *sig = spl + 3L * pr.kc_off;
+    const double *pv = sig + n;
+    const double *gam = pv + n;
+    for (int col = threadIdx.x; col < tp2 * nt; col += LOS_SPF_THREADS) {
+      const int tp = col / nt, t = col - tp * nt;
+      double *y = dv + (tp * TB + t) * ls;  // y_i, then u_i, then the second derivatives
+      const int a = n - 3, b = n - 2, e = n - 1;
+      const double ya = y[a], yb = y[b], ye = y[e];
+      const double y0 = y[0], y1 = y[1], y2 = y[2];
+      double d0 = ((x[2] - x[0]) * (x[2] - x[0]) * (y1 - y0) - (x[1] - x[0]) * (x[1] - x[0]) * (y2 - y0)) /
+                  ((x[2] - x[0]) * (x[1] - x[0]) * (x[2] - x[1]));
+      double u = (3.0 / (x[1] - x[0])) * ((y1 - y0) / (x[1] - x[0]) - d0);
+      y[0] = u;
+      double ym = y0, yc = y1;
+      for (int i = 1; i < n - 1; ++i) {
+        const double yn = y[i + 1];
+        double tt = (yn - yc) / (x[i + 1] - x[i]) - (yc - ym) / (x[i] - x[i - 1]);
+        u = (6.0 * tt / (x[i + 1] - x[i - 1]) - sig[i] * u) / pv[i];
+        y[i] = u;
+        ym = yc;
+        yc = yn;
+      }
+      double dn = ((x[e] - x[a]) * (x[e] - x[a]) * (ye - yb) - (x[e] - x[b]) * (x[e] - x[b]) * (ye - ya)) /
+                  ((x[e] - x[a]) * (x[e] - x[b]) * (x[b] - x[a]));
+      double qn = 0.5;
+      double un = (3.0 / (x[e] - x[b])) * (dn - (ye - yb) / (x[e] - x[b]));
+      double next = (un - qn * u) / (qn * gam[n - 2] + 1.0);
+      y[e] = next;
+      for (int k = n - 2; k >= 0; --k) {
+        next = gam[k] * next + y[k];
+        y[k] = next;
+      }
+    }
+    __syncthreads();
+  }
+
+  // records of every fine k3 of the pair: [f][t][mi][field][L], t in the chunk
+  const int nmc = P.m_size * 3;
+  const int rec = nmc * NL;
+  const int per_f = nt * rec;
+  const double *kcg = P.kc + pr.kc_off;
+  R *sf = (R *)P.sf;
+  for (int q = threadIdx.x; q < pr.nf * per_f; q += LOS_SPF_THREADS) {
+    const int fl = q / per_f, r = q - fl * per_f;
+    const int t = r / rec, w = r - t * rec;
+    const int mc = w / NL, L = w - mc * NL;
+    const int mi = mc / 3, c = mc - mi * 3;
+    int tp = -1;
+    if (c == 0 && P.has_T && L <= P.LmaxT) tp = P.srcT[L][mi];
+    if (c == 1 && P.has_E && L <= P.LmaxP) tp = P.srcE[L][mi];
+    if (c == 2 && P.has_B && L <= P.LmaxP) tp = P.srcB[L][mi];
+    const int f = pr.f_off + fl;
+    const int br = P.fbr[f];
+    const double ks = P.kf[P.fsrc[f]];
+    const double h = kcg[br + 1] - kcg[br];
+    const double bw = (ks - kcg[br]) / h;
+    const double aw = 1 - bw;
+    const double ca = aw * aw * aw - aw, cb = bw * bw * bw - bw;
+    const int row = (max(tp, 0) * TB + t) * ls;
+    const double s0 = yv[row + br], s1 = yv[row + br + 1];
+    double v;
+    if (cubic) {
+      const double d0 = dv[row + br], d1 = dv[row + br + 1];
+      v = aw * s0 + bw * s1 + (ca * d0 + cb * d1) * h * h / 6.0;
+    } else {
+      v = aw * s0 + bw * s1;
+    }
+    if (WITH_DT) v = v * P.dtau[t0 + t];
+    sf[((long)f * ntau + t0) * rec + r] = (R)(tp >= 0 ? v : 0.0);
+  }
+  for (int q = threadIdx.x; q < pr.nf * nt; q += LOS_SPF_THREADS) {
+    const int fl = q / nt, t = q - fl * nt;
+    const int f = pr.f_off + fl;
+    const double k = P.kf[f];
+    const double x = k * P.t0mt[t0 + t];
+    const int ix = (int)(x / P.xx_step);
+    SweepIXA<R> wv;
+    wv.ix = ix;
+    wv.a = (R)((P.xx[ix + 1] - x) / P.xx_step);
+    ((SweepIXA<R> *)P.ixa)[(long)f * ntau + t0 + t] = wv;
+  }
+}
+
+// Steps per workgroup of los_sweep_prep_fused: as many as fit in 64 KiB of LDS
+// (values and second derivatives of every source type), 1 to 8.
+static inline int sweep_fused_steps(int tp2, int ls)
+{
+  const long per = 2L * tp2 * ls * (long)sizeof(double);
+  long tb = per > 0 ? (64L * 1024) / per : 8;
+  return (int)(tb < 1 ? 1 : tb > 8 ? 8 : tb);
+}
+
+// Whether los_sweep_prep_fused can take a batch: one step of every source type
+// (values and second derivatives) must fit in 64 KiB of LDS.
+static inline bool sweep_fused_fits(int tp2, int nc_max)
+{
+  return 2L * tp2 * (nc_max | 1) * (long)sizeof(double) <= 64L * 1024;
+}
+
+// Tables of the pairs p_begin .. p_end - 1 from the raw sources.
+template <class R, bool WITH_DT>
+static hipError_t launch_sweep_prep_fused(const los_params *P, const double *raw, const double *spl,
+                                          int nc_max, int p_begin, int p_end, hipStream_t s)
+{
+  if (P->n_pairs == 0 || p_end <= p_begin || P->ntau == 0) return hipSuccess;
+  if (!sweep_fused_fits(P->tp2_size, nc_max)) return hipErrorInvalidValue;
+  const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
+  const int ls = nc_max | 1;
+  const int TB = sweep_fused_steps(P->tp2_size, ls);
+  const size_t lds = 2 * (size_t)P->tp2_size * TB * ls * sizeof(double);
+  const int nchunk = (P->ntau + TB - 1) / TB;
+  const dim3 grid((unsigned)((long)(p_end - p_begin) * nchunk)), block(LOS_SPF_THREADS);
+  switch (NL) {
+#define LOS_SPF(k)                                                                             \
+  case k:                                                                                     \
+    hipLaunchKernelGGL((los_sweep_prep_fused<R, WITH_DT, k>), grid, block, lds, s, *P, raw, spl, \
+                       p_begin, nchunk, TB, ls);                                              \
+    break;
+    LOS_SPF(1) LOS_SPF(2) LOS_SPF(3) LOS_SPF(4) LOS_SPF(5) LOS_SPF(6) LOS_SPF(7) LOS_SPF(8)
+#undef LOS_SPF
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Sweep kernel. Block = one wave; blockIdx.y = wave class (mi, l chunk);
 // blockIdx.x = XCD-aware tile index (tiles of XCD b % 8, costliest first).
 // ---------------------------------------------------------------------------

@@ -118,6 +118,7 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_load_pairs": (i, [vp] * 7 + [C.POINTER(i), C.POINTER(i), i, ch]),
         "los_engine_autotune": (i, [vp, i, ch]),
         "los_engine_set_pipeline": (i, [vp, i, ch]),
+        "los_engine_set_fused_prep": (i, [vp, i, ch]),
         "los_engine_set_shape": (i, [vp, i, i, i, i, i, ch]),
         "los_engine_get_shape": (None, [vp, i] + [C.POINTER(i)] * 4),
         "los_engine_capture": (i, [vp, ch]),
@@ -450,6 +451,11 @@ class Engine:
         """Pipelined steps (chunked preparation overlapping the LOS kernel): n >= 2 chunks
         from the next load; 0 or 1 off (the default)."""
         self._call("los_engine_set_pipeline", int(on))
+
+    def set_fused_prep(self, on: bool) -> None:
+        """Sweep tables straight from the raw sources (default) or through re-laid and
+        splined sources in HBM."""
+        self._call("los_engine_set_fused_prep", int(bool(on)))
 
     def autotune(self, reps: int = 2) -> None:
         self._call("los_engine_autotune", reps)
