@@ -101,6 +101,12 @@ int los_engine_run(los_engine *eng, float *ms, char *errmsg);
    No counterpart in SONG. */
 int los_engine_set_pipeline(los_engine *e, int on, char *errmsg);
 
+/* Sweep shapes: with on != 0 (the default), the per-(k3, tau) tables are made
+   straight from the sources as SONG lays them out, splined in LDS, when one
+   time step of every source type fits in 64 KiB of LDS; with on == 0 the
+   sources are first re-laid out and splined in HBM. Same values either way. */
+int los_engine_set_fused_prep(los_engine *e, int on, char *errmsg);
+
 /* GPU time of the last eager run's LOS kernel and of everything before it (source
    re-layout, k3 spline, per-(k3,tau) tables of the kernel shape), in ms; -1 after
    a graph or pipelined run (only the total is known). */
@@ -406,6 +412,7 @@ int los_cl2_create(const los_cl2_desc *desc, int device, los_cl2 **out, char *er
 int los_cl2_run(los_cl2 *h, int reps, float *ms, char *errmsg);
 int los_cl2_result(los_cl2 *h, double *cl_m, long n, char *errmsg);
 void los_cl2_destroy(los_cl2 *h);
+
 
 #ifdef __cplusplus
 }
