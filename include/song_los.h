@@ -101,9 +101,9 @@ int los_engine_run(los_engine *eng, float *ms, char *errmsg);
    No counterpart in SONG. */
 int los_engine_set_pipeline(los_engine *e, int on, char *errmsg);
 
-/* Sweep shapes: with on != 0 (the default), the per-(k3, tau) tables are made
-   straight from the sources as SONG lays them out, splined in LDS, when one
-   time step of every source type fits in 64 KiB of LDS; with on == 0 the
+/* Sweep shapes: with on != 0, the per-(k3, tau) tables are made straight from
+   the sources as SONG lays them out, splined in LDS, when one time step of
+   every source type fits in 64 KiB of LDS; with on == 0 (the default) the
    sources are first re-laid out and splined in HBM. Same values either way. */
 int los_engine_set_fused_prep(los_engine *e, int on, char *errmsg);
 

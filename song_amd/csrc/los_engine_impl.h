@@ -113,8 +113,10 @@ struct los_engine {
   los_params P;
   bool loaded = false;
   // sweep tables straight from the raw sources (los_sweep_prep_fused) when the
-  // batch fits its LDS budget and fused_prep is on (the default)
-  bool fused_prep = true;
+  // batch fits its LDS budget and fused_prep is on (off by default: on c2 it
+  // took 18.7 ms against 16.5 ms for re-layout + spline + tables in two passes,
+  // its 504-byte records being written from 316,800 one-step workgroups)
+  bool fused_prep = false;
   bool pipeline = false;  // pipelined steps (chunks_forced >= 2 and a sweep shape)
 
   double *staging = nullptr;  // pinned

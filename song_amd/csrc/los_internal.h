@@ -6,8 +6,16 @@
 #ifndef SONG_LOS_INTERNAL_H
 #define SONG_LOS_INTERNAL_H
 
+/* Built into SONG (-DLOS_SONG_HEADERS with SONG's include/ and CLASS's on the
+   include path), the host units read SONG's own structure definitions;
+   otherwise this repository's (same fields, the subset the stage uses). */
+#ifdef LOS_SONG_HEADERS
+#include "common2.h"
+#include "transfer2.h"
+#else
 #include "../../include/song_los_types.h"
 #include "../../include/transfer2.h"
+#endif
 #include "los_common.h"
 
 #ifdef __cplusplus
