@@ -413,6 +413,21 @@ int los_cl2_run(los_cl2 *h, int reps, float *ms, char *errmsg);
 int los_cl2_result(los_cl2 *h, double *cl_m, long n, char *errmsg);
 void los_cl2_destroy(los_cl2 *h);
 
+/* The C_l sum of los_cl2_create + los_cl2_run + los_cl2_result in one call
+   that allocates nothing. No counterpart in the reference: it is for callers
+   that cannot allocate device memory (code under HIP graph capture, serving
+   loops). Every pointer of `desc` and cl_m ([n_l][m_size]) are device
+   pointers of the current HIP device; n_k3 = pair_off[k_size (k_size+1)/2];
+   the work is enqueued on `stream` (a hipStream_t; NULL for the default
+   stream) and not waited for. The inputs must satisfy what los_cl2_create
+   checks (two or more k3 points per pair, positive k and k3, m >= 0). Limits:
+   k_size <= 128, n_k3 <= 2^21, n_l * m_size <= 8192 (larger inputs are
+   refused with a message). The per-pair tables and partial sums use a fixed
+   10.4 MiB device array per GPU, held until the process ends and shared by
+   every call on that GPU: calls that overlap on one GPU race. Same kernels as
+   los_cl2_run, so the results are the same bit for bit. */
+int los_cl2_sum_noalloc(const los_cl2_desc *desc, long n_k3, double *cl_m, void *stream,
+                        char *errmsg);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "los_common.h"
@@ -286,6 +287,119 @@ int los_cl2_result(los_cl2 *h, double *cl_m, long n, char *errmsg)
   if (rows == 0) return _SUCCESS_;
   CCHK(hipSetDevice(h->device));
   CCHK(hipMemcpy(cl_m, h->out, (size_t)rows * sizeof(double), hipMemcpyDeviceToHost));
+  return _SUCCESS_;
+}
+
+
+// ---------------------------------------------------------------------------
+// The same sum for callers that cannot allocate device memory: every input and
+// the output are the caller's device arrays; the per-pair table, the pair of
+// each k3 point and the partial sums live in `scratch`, a file-scope device
+// array (one per GPU, held until the process ends). Each kernel writes every
+// scratch value it later reads, so nothing needs clearing between calls;
+// calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+#define CL2_NA_KSIZE_MAX 128                                   // k grid points
+#define CL2_NA_PAIRS_MAX (CL2_NA_KSIZE_MAX * (CL2_NA_KSIZE_MAX + 1) / 2)
+#define CL2_NA_K3_MAX (1L << 21)                               // k3 points of all pairs
+#define CL2_NA_ROWS_MAX 8192L                                  // (l, m) rows
+#define CL2_NA_OFF_EDGE (((6L * CL2_NA_PAIRS_MAX * 8) + 255) / 256 * 256)
+#define CL2_NA_OFF_PAIR (CL2_NA_OFF_EDGE + ((2L * CL2_NA_PAIRS_MAX * 4) + 255) / 256 * 256)
+#define CL2_NA_OFF_PART (CL2_NA_OFF_PAIR + ((CL2_NA_K3_MAX * 4) + 255) / 256 * 256)
+#define CL2_NA_BYTES (CL2_NA_OFF_PART + CL2_NA_ROWS_MAX * CL2_CHUNKS * 8)
+
+static __device__ __align__(256) unsigned char scratch[CL2_NA_BYTES];
+
+// Per pair (one thread each): k1, k2, half steps and P values, the two
+// triangle-edge points and the pair of each of its k3 points, with the host
+// expressions of los_cl2_create (this unit is compiled without contraction).
+__global__ void __launch_bounds__(64)
+los_cl2_pairs(int k_size, const double *__restrict__ k, const double *__restrict__ pk,
+              const long *__restrict__ pair_off, const double *__restrict__ k3g,
+              double *__restrict__ pq, int *__restrict__ edge, int *__restrict__ pair_of)
+{
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  if (p >= k_size * (k_size + 1) / 2) return;
+  int i1 = (int)((sqrt(8.0 * p + 1.0) - 1.0) / 2.0);
+  while (i1 * (i1 + 1) / 2 > p) --i1;
+  while ((i1 + 1) * (i1 + 2) / 2 <= p) ++i1;
+  const int i2 = p - i1 * (i1 + 1) / 2;
+  const int ks = k_size;
+  double s1, s2;
+  if (i1 == 0) s1 = (k[1] - k[0]) / 2.;
+  else if (i1 == ks - 1) s1 = (k[ks - 1] - k[ks - 2]) / 2.;
+  else s1 = (k[i1 + 1] - k[i1 - 1]) / 2.;
+  if (i1 == 0) s2 = 0.;
+  else if (i2 == 0) s2 = (k[1] - k[0]) / 2.;
+  else if (i2 == i1) s2 = (k[i1] - k[i1 - 1]) / 2.;
+  else s2 = (k[i2 + 1] - k[i2 - 1]) / 2.;
+  const double k1 = k[i1], k2 = k[i2];
+  const double PI = M_PI;
+  double *q = pq + 6 * (long)p;
+  q[0] = k1;
+  q[1] = k2;
+  q[2] = s1;
+  q[3] = s2;
+  q[4] = 2 * PI * PI / (k1 * k1 * k1) * pk[i1];
+  q[5] = 2 * PI * PI / (k2 * k2 * k2) * pk[i2];
+  const long b = pair_off[p];
+  const int n = (int)(pair_off[p + 1] - b);
+  const double *g = k3g + b;
+  int first = -1, last = -1;
+  for (int j = 0; j < n; ++j) {
+    if (first < 0 && g[j] >= k1 - k2 && j < n - 1) first = j;
+    const bool next_out = (j == n - 1) || (g[j + 1] > k1 + k2);
+    if (last < 0 && next_out && j > 0) last = j;
+    pair_of[b + j] = p;
+  }
+  edge[2 * p] = first;
+  edge[2 * p + 1] = last;
+}
+
+namespace {
+std::mutex na_mutex;
+void *na_base[64];  // device address of `scratch` on each device, once looked up
+}  // namespace
+
+extern "C" int los_cl2_sum_noalloc(const los_cl2_desc *d, long n_k3, double *cl_m, void *stream,
+                                   char *errmsg)
+{
+  los_test(d->k_size < 2 || d->k_size > CL2_NA_KSIZE_MAX, errmsg,
+           "k_size = %d: 2 to %d points", d->k_size, CL2_NA_KSIZE_MAX);
+  los_test(n_k3 < 0 || n_k3 > CL2_NA_K3_MAX, errmsg, "n_k3 = %ld: at most %ld k3 points", n_k3,
+           (long)CL2_NA_K3_MAX);
+  const long rows = (long)d->n_l * d->m_size;
+  los_test(d->n_l < 0 || d->m_size < 1 || rows > CL2_NA_ROWS_MAX, errmsg,
+           "%d l x %d m: at most %ld (l, m) rows", d->n_l, d->m_size, (long)CL2_NA_ROWS_MAX);
+  int dev = 0;
+  CCHK(hipGetDevice(&dev));
+  los_test(dev < 0 || dev >= 64, errmsg, "device %d", dev);
+  void *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(na_mutex);
+    if (na_base[dev] == nullptr) {
+      void *ptr = nullptr;
+      CCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      na_base[dev] = ptr;
+    }
+    base = na_base[dev];
+  }
+  if (rows == 0) return _SUCCESS_;
+  unsigned char *b = (unsigned char *)base;
+  double *pq = (double *)b;
+  int *edge = (int *)(b + CL2_NA_OFF_EDGE);
+  int *pair_of = (int *)(b + CL2_NA_OFF_PAIR);
+  double *partial = (double *)(b + CL2_NA_OFF_PART);
+  hipStream_t s = (hipStream_t)stream;
+  const int n_pairs = d->k_size * (d->k_size + 1) / 2;
+  hipLaunchKernelGGL(los_cl2_pairs, dim3((n_pairs + 63) / 64), dim3(64), 0, s, d->k_size, d->k,
+                     d->pk, d->pair_off, d->k3, pq, edge, pair_of);
+  hipLaunchKernelGGL(los_cl2_partial, dim3((unsigned)(rows * CL2_CHUNKS)), dim3(CL2_THREADS), 0,
+                     s, d->pair_off, pair_of, edge, pq, d->k3, n_k3, d->transfer1, d->transfer2,
+                     d->m, d->m_size, d->used, d->rescale, partial);
+  hipLaunchKernelGGL(los_cl2_finish, dim3((unsigned)((rows + 63) / 64)), dim3(64), 0, s, partial,
+                     rows, cl_m);
+  CCHK(hipGetLastError());
   return _SUCCESS_;
 }
 
