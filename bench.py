@@ -322,7 +322,7 @@ def serve_bench(args, cfg):
         res = {"capacity_jobs_per_s": cap, "rates": []}
         for frac in (0.5, 0.8, 1.1):
             rate = frac * cap
-            n = int(min(300, max(40, rate * args.serve_seconds)))
+            n = int(min(args.serve_jobs, max(40, rate * args.serve_seconds)))
             jobs = make_jobs(n)
             gaps = rng.exponential(1.0 / rate, n)
             ids = [None] * n
@@ -352,7 +352,9 @@ def serve_bench(args, cfg):
             el = sum(j[2] for j in jobs)
             lat = np.array(lat)
             r = {"offered_jobs_per_s": rate, "jobs": n, "p50_ms": float(np.percentile(lat, 50)),
-                 "p99_ms": float(np.percentile(lat, 99)), "elements_per_s": el / (t_last - t_first),
+                 "p99_ms": float(np.percentile(lat, 99)),
+                 "p99_samples": int(np.count_nonzero(lat >= np.percentile(lat, 99))),
+                 "elements_per_s": el / (t_last - t_first),
                  "launches": st["launches"] - st0["launches"],
                  "max_busy_slots": st["max_busy_slots"]}
             print(f"# {form} rate {rate:.1f}/s: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms "
@@ -622,6 +624,8 @@ def main(argv=None):
                     help="c5: HBM budget of the admitted jobs, GiB")
     ap.add_argument("--serve-seconds", type=float, default=15.0,
                     help="c5: arrival span per rate, seconds")
+    ap.add_argument("--serve-jobs", type=int, default=1000,
+                    help="c5: at most this many jobs per rate (p99 rests on 1% of them)")
     args = ap.parse_args(argv)
     # CPU-baseline protocol: threads bound close, one per core (read by the OpenMP runtime
     # when it starts, which is when the first library that uses it loads)

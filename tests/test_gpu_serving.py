@@ -113,3 +113,50 @@ def test_budget_admission(scen_ref):
         allp = [(i1, i2) for i1 in range(CFG["k_size"]) for i2 in range(i1 + 1)]
         srv.submit(allp, 600)  # needs more than the whole budget
     srv.close()
+
+
+OVERLAP = dict(has_T=1, has_E=1, m_list=[0, 1, 2], l_max_los_t=6, l_max_los_p=6, k_size=16,
+               k_min=1e-5, k_max=0.1, k3_size=60, tau_size=600, l_size=96, l_max=2000,
+               dk_fine=1.5e-4, xx_max=1500.0)
+
+
+@pytest.mark.gpu
+def test_jobs_arriving_during_a_launch(oracle):
+    """Continuous batching: small jobs submitted while a long launch runs are admitted to
+    other slots and finish before it (their admission precedes its completion, and at
+    least two slots were busy at once); each equals its result alone and the oracle's rows
+    bit for bit (plain FP64 form)."""
+    import time
+    from tests.conftest import oracle_pairs
+
+    cfg = L.Config(**OVERLAP)
+    s = L.Scenario(cfg)
+    s.prepare()
+    allp = list(s.pair_offsets())
+    small = [([allp[5]], 500), ([allp[40], allp[7]], 1000), ([allp[-1]], 2000)]
+    want = oracle_pairs(s, oracle, sorted({p for pp, _ in small for p in pp}))
+    lm_l = s.lm_l()
+    n_all = len(lm_l)
+
+    def expected(pp, lmax):
+        n_lm = max(i for i, l in enumerate(lm_l) if l <= lmax) + 1
+        return np.array([np.concatenate([want[p][f * n_all + q] for p in pp])
+                         for f in range(3) for q in range(n_lm)])
+
+    srv = L.Server(s, L.FORM_FP64, budget_bytes=16 * 2.0 ** 30, slots=3)
+    alone = [srv.wait(srv.submit(pp, lm), 120000)[0] for pp, lm in small]
+    big = srv.submit(allp, 2000)
+    time.sleep(0.005)  # the long job is admitted and running
+    ids = [srv.submit(pp, lm) for pp, lm in small]
+    res = [srv.wait(i, 120000) for i in ids]
+    _, (_, _, big_done) = srv.wait(big, 120000)
+    st = srv.stats()
+    admits = [t[1] for _, t in res]
+    print(f"big done at {big_done:.4f}; small admitted at {admits}; stats {st}", flush=True)
+    assert st["max_busy_slots"] >= 2
+    assert min(admits) < big_done
+    for (pp, lm), a, (got, _) in zip(small, alone, res):
+        assert np.array_equal(got, a)
+        assert np.array_equal(got, expected(pp, lm))
+    srv.close()
+    s.close()
