@@ -274,6 +274,8 @@ typedef struct los_scenario los_scenario;
 
 int los_scenario_create(const los_config *cfg, los_scenario **out, char *errmsg);
 void los_scenario_destroy(los_scenario *s);
+int los_scenario_override_ddJ(los_scenario *s, int J, int L, int il, int im, const double *v,
+                              int n, char *errmsg);
 void los_scenario_structs(los_scenario *s, struct precision **ppr, struct precision2 **ppr2,
                           struct background **pba, struct thermo **pth,
                           struct perturbs **ppt, struct perturbs2 **ppt2,

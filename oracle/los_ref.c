@@ -45,7 +45,8 @@
    tests can show they catch it (1: k3 spline weights a and b swapped; 2: the spline's
    h^2/6 term as h^2/3; 3: the J skip as index_x <= index_xmin_J; 4: the flat k3
    extrapolation fills swapped; 5: linear k3 weights swapped; 6: linear time weights
-   swapped). The oracle proper is built with LOS_PLANT = 0. */
+   swapped; 7: the cubic J weights of ddJ_left and ddJ_right swapped). The oracle proper is
+   built with LOS_PLANT = 0. */
 #ifndef LOS_PLANT
 #define LOS_PLANT 0
 #endif
@@ -694,8 +695,13 @@ static int ref_integrate(struct precision *ppr, struct precision2 *ppr2, struct 
         double *ddJ = &(pbs2->ddJ_Llm_x[index_J][iL][index_l][index_m][ixJ]);
         double ddl = *ddJ;
         double ddr = *(ddJ + 1);
+#if LOS_PLANT == 7
+        J_Llm = (a_J * Jl + (1. - a_J) * (Jr - a_J * ((2. - a_J) * ddl + (a_J + 1.) * ddr) *
+                                                     pbs2->xx_step * pbs2->xx_step / 6.0));
+#else
         J_Llm = (a_J * Jl + (1. - a_J) * (Jr - a_J * ((a_J + 1.) * ddl + (2. - a_J) * ddr) *
                                                      pbs2->xx_step * pbs2->xx_step / 6.0));
+#endif
       }
       double source_Lm = src[index_source_monopole + ppt2->lm_array[L][ppr2->index_m[m]]][it];
       integrand += J_Llm * source_Lm;

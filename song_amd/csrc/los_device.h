@@ -113,6 +113,8 @@ typedef struct {
      m and 64 consecutive l: lane slot lp = mi * n_lp + il (n_lp = l count
      rounded up to 64). */
   const void *Jp;        /* J re-laid out [jt][L][x][lp] (FP64 pairs or FP32) */
+  const void *Jd;        /* second derivatives of J as FP64 pairs, same layout (cubic J) */
+  int cubic_J;           /* 1: cubic interpolation of J in x (plain FP64 sweep kernels) */
   unsigned Jp_bytes;     /* size of the table at Jp (read with 32-bit byte offsets) */
   const int *xminp;      /* [jt][L][lp] first live x index, INT_MAX for empty slots */
   const int *lm_of_lp;   /* (l,m) offset of each slot, -1 for empty slots */

@@ -102,6 +102,10 @@ static void free_tables(los_engine *e)
   dfree(e->dmi_of);
   dfree(e->dJp);
   dfree(e->dJp32);
+  dfree(e->dDp);
+  if (e->dDp2) (void)hipFree(e->dDp2);
+  e->dDp2 = nullptr;
+  e->tables_cubic = false;
   for (int i = 0; i < 2; ++i) {
     if (e->dJp2[i]) (void)hipFree(e->dJp2[i]);
     e->dJp2[i] = nullptr;
@@ -194,6 +198,7 @@ static int build_tables(los_engine *e, const los_stage *st, double kmax_all, cha
   const int X = std::min(ixmax + 2, st->xx_size);
   const size_t nJ = (size_t)J_size * L_size * X * n_lm;
   std::vector<double> hJ(nJ, 0.0);
+  std::vector<double> hD(st->ddJ ? nJ : 0, 0.0);  // second derivatives (cubic J)
   std::vector<int> hxmin((size_t)J_size * L_size * n_lm, INT_MAX);
   double window = 0;
   for (int jt = 0; jt < J_size; ++jt)
@@ -211,6 +216,11 @@ static int build_tables(los_engine *e, const los_stage *st, double kmax_all, cha
           const int xend = std::min(X, xm + xs);
           for (int x = xm; x < xend; ++x)
             hJ[(((size_t)jt * L_size + L) * X + x) * n_lm + lm] = row[x - xm];
+          if (st->ddJ) {
+            const double *drow = st->ddJ[jt][L][il][im];
+            for (int x = xm; x < xend; ++x)
+              hD[(((size_t)jt * L_size + L) * X + x) * n_lm + lm] = drow[x - xm];
+          }
           window += 8.0 * (xend - xm);
         }
       }
@@ -247,7 +257,7 @@ static int build_tables(los_engine *e, const los_stage *st, double kmax_all, cha
     for (int im = 0; im < m_size; ++im)
       if (im <= st->index_m_max[st->l[il]]) lm_of_lp[im * n_lp + il] = st->lm_array[il][im];
   const size_t nJp = (size_t)J_size * L_size * X * NLP;
-  std::vector<double> hJp(nJp, 0.0);
+  std::vector<double> hJp(nJp, 0.0), hDp(st->ddJ ? nJp : 0, 0.0);
   std::vector<int> hxminp((size_t)J_size * L_size * NLP, INT_MAX);
   for (size_t r = 0; r < (size_t)J_size * L_size; ++r)
     for (int lp = 0; lp < NLP; ++lp) {
@@ -255,6 +265,8 @@ static int build_tables(los_engine *e, const los_stage *st, double kmax_all, cha
       if (lm < 0) continue;
       hxminp[r * NLP + lp] = hxmin[r * n_lm + lm];
       for (int x = 0; x < X; ++x) hJp[(r * X + x) * NLP + lp] = hJ[(r * X + x) * n_lm + lm];
+      if (st->ddJ)
+        for (int x = 0; x < X; ++x) hDp[(r * X + x) * NLP + lp] = hD[(r * X + x) * n_lm + lm];
     }
   e->wave_minx.assign(NLP / 64, INT_MAX);
   for (int lp = 0; lp < NLP; ++lp)
@@ -295,6 +307,10 @@ static int build_tables(los_engine *e, const los_stage *st, double kmax_all, cha
   }
   HIPCHK(hipMalloc((void **)&e->dJp, nJp * sizeof(double)), err);
   if (upload(e, e->dJp, hJp.data(), nJp * sizeof(double), err)) return _FAILURE_;
+  if (st->ddJ) {
+    HIPCHK(hipMalloc((void **)&e->dDp, nJp * sizeof(double)), err);
+    if (upload(e, e->dDp, hDp.data(), nJp * sizeof(double), err)) return _FAILURE_;
+  }
   if (upload_vec(e, e->dxminp, hxminp, err) || upload_vec(e, e->dlm_of_lp, lm_of_lp, err))
     return _FAILURE_;
   e->minx_all = INT_MAX;
@@ -314,7 +330,8 @@ static int build_tables(los_engine *e, const los_stage *st, double kmax_all, cha
   e->n_lm = n_lm;
   e->nJ = nJ;
   e->J_owner = st->J_id;
-  e->J_window_bytes = window;
+  e->tables_cubic = st->ddJ != nullptr;
+  e->J_window_bytes = window * (st->ddJ ? 2 : 1);
   return _SUCCESS_;
 }
 
@@ -361,9 +378,10 @@ extern "C" int los_engine_load_stage(los_engine *e, const los_stage *st, const i
              "los_engine_load: only transfer2_tau_sampling=sources is supported by this build");
     return _FAILURE_;
   }
-  if (!st->linear_J) {
+  if (!st->linear_J && st->ddJ == nullptr) {
     snprintf(err, _ERRORMSGSIZE_,
-             "los_engine_load: only linear bessels_interpolation is supported by this build");
+             "los_engine_load: cubic bessels_interpolation needs the second derivatives of J "
+             "(pbs2->ddJ_Llm_x)");
     return _FAILURE_;
   }
   if (st->tp2_size <= 0 || st->tau_size < 2) {
@@ -518,7 +536,8 @@ extern "C" int los_engine_load_stage(los_engine *e, const los_stage *st, const i
   e->ntau = ntau;
 
   // projection tables: built once for the whole k range of the run
-  if (e->J_owner != st->J_id) {
+  e->cubic_J = !st->linear_J;
+  if (e->J_owner != st->J_id || e->tables_cubic != e->cubic_J) {
     double kmax_all = 0;
     for (int i1 = 0; i1 < st->k_size; ++i1)
       for (int i2 = 0; i2 <= i1; ++i2) kmax_all = std::max(kmax_all, st->k_max_k1k2[i1][i2]);
@@ -741,8 +760,11 @@ static int form_shapes(const los_engine *e, int form, los_shape *out, int cap)
     n = los_shapes(form == LOS_FORM_FP32C, e->loaded ? e->P.tp2_size : 1, all, 48);
   }
   int k = 0;
-  for (int i = 0; i < n && k < cap; ++i)
+  for (int i = 0; i < n && k < cap; ++i) {
+    // cubic interpolation of J: the plain FP64 sweep kernels only
+    if (e->cubic_J && (form != LOS_FORM_FP64 || all[i].kind != 1)) continue;
     if (all[i].kind != 1 || sweep_fits(e, form, all[i])) out[k++] = all[i];
+  }
   return k;
 }
 
@@ -774,6 +796,8 @@ int enqueue_los_params(los_engine *e, const los_shape *sh, los_params *P, hipStr
   }
   if (e->form == LOS_FORM_FP64 && sh->kind == 1) {
     P->Jp = e->dJp2[0];
+    P->Jd = e->dDp2;
+    P->cubic_J = e->cubic_J ? 1 : 0;
     P->Jp_bytes = (unsigned)(e->nJp * 2 * sizeof(double));
     split_xcd(tiles, cost, P->xcd_tile);
     HIPCHK(los_launch_sweep_exact(P, sh->nk, sh->persistent, e->n_cu, s), err);
@@ -925,6 +949,15 @@ int ensure_sweep(los_engine *e, const los_shape *sh, char *err)
            err);
     HIPCHK(hipStreamSynchronize(e->stream), err);
   }
+  if (!fp32 && e->cubic_J && e->dDp2 == nullptr) {
+    // second derivatives as (dd[x], dd[x+1]) pairs with the same zeroing
+    const int NLP = e->m_size * e->n_lp;
+    HIPCHK(hipMalloc(&e->dDp2, std::max<size_t>(1, e->nJp) * 2 * sizeof(double)), err);
+    HIPCHK(los_launch_sweep_pairs(0, e->dDp, e->dDp2, e->dxminp, (long)e->J_size * e->L_size,
+                                  e->X, NLP, e->stream),
+           err);
+    HIPCHK(hipStreamSynchronize(e->stream), err);
+  }
   if (fp32 && sh->ch == 1 && e->dJp2[1] == nullptr) {
     // FP32 pairs of the rounded table, same zeroing
     const int NLP = e->m_size * e->n_lp;
@@ -999,7 +1032,7 @@ int ensure_j32(los_engine *e, char *err)
 
 // Autotuner cache: best shape per (form, (l,m) count, source types, sources L,
 // time steps, fine-k3 count rounded to a power of two).
-typedef std::tuple<int, int, int, int, int, int> tune_key;
+typedef std::tuple<int, int, int, int, int, int, int> tune_key;
 static std::map<tune_key, los_shape> g_tuned;
 static std::mutex g_tuned_mu;
 
@@ -1007,7 +1040,8 @@ static tune_key key_of(const los_engine *e)
 {
   int fb = 0;
   while ((1 << fb) < e->F) ++fb;
-  return tune_key(e->form, e->n_lm, e->P.tp2_size, std::max(e->P.LmaxT, e->P.LmaxP), e->ntau, fb);
+  return tune_key(e->form, e->n_lm, e->P.tp2_size, std::max(e->P.LmaxT, e->P.LmaxP), e->ntau, fb,
+                  e->cubic_J ? 1 : 0);
 }
 
 extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
@@ -1037,8 +1071,12 @@ extern "C" int los_engine_autotune(los_engine *e, int reps, char *err)
   for (int i = 0; i < na; ++i)
     if (tuned_shape(all[i])) cand[nc++] = all[i];
   if (nc == 0) {
-    snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: no tile shape fits %d source types in LDS",
-             e->P.tp2_size);
+    if (e->cubic_J)
+      snprintf(err, _ERRORMSGSIZE_, "los_engine_autotune: cubic bessels_interpolation is "
+               "implemented in the plain FP64 form (LOS_FORM_FP64) only");
+    else
+      snprintf(err, _ERRORMSGSIZE_,
+               "los_engine_autotune: no tile shape fits %d source types in LDS", e->P.tp2_size);
     return _FAILURE_;
   }
   float best = 1e30f;
@@ -1112,6 +1150,11 @@ int current_shape(los_engine *e, const los_shape **sh, char *err)
   if (!e->shape_set[e->form])
     if (los_engine_autotune(e, 2, err)) return _FAILURE_;
   *sh = &e->shape[e->form];
+  if (e->cubic_J && (e->form != LOS_FORM_FP64 || (*sh)->kind != 1)) {
+    snprintf(err, _ERRORMSGSIZE_, "cubic bessels_interpolation is implemented in the plain FP64 "
+             "form's sweep kernels only (form %d, kernel kind %d)", e->form, (*sh)->kind);
+    return _FAILURE_;
+  }
   return ensure_tables(e, *sh, err);
 }
 

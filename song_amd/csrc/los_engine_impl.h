@@ -81,6 +81,11 @@ struct los_engine {
   float *dJp32 = nullptr;
   size_t nJp = 0;
   void *dJp2[2] = {nullptr, nullptr};  // (J[x], J[x+1]) pair tables, FP64 and FP32
+  // cubic interpolation of J in x (plain FP64 sweep kernels only): second
+  // derivatives in the sweep layout, and as (dd[x], dd[x+1]) pairs
+  bool cubic_J = false, tables_cubic = false;
+  double *dDp = nullptr;
+  void *dDp2 = nullptr;
   int *dxminp = nullptr, *dlm_of_lp = nullptr;
   int n_lp = 0, n_wave = 0, m_size = 0;
   std::vector<int> wave_minx;  // smallest first-live x index over the lanes of each wave class

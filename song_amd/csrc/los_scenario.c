@@ -541,6 +541,21 @@ int los_scenario_override_J(los_scenario *s, int J, int L, int il, int im, int x
   return _SUCCESS_;
 }
 
+/* Replace the second derivatives of one J row (cubic J interpolation); the row
+   keeps its first index and size. */
+int los_scenario_override_ddJ(los_scenario *s, int J, int L, int il, int im, const double *v,
+                              int n, char *err)
+{
+  struct bessels2 *b = &s->bs2;
+  los_test(b->ddJ_Llm_x == NULL, err, "no second derivatives: J interpolation is linear");
+  los_test(n != b->x_size_J[J][L][il][im], err, "the row holds %d values, not %d",
+           b->x_size_J[J][L][il][im], n);
+  free(b->ddJ_Llm_x[J][L][il][im]);
+  los_alloc(b->ddJ_Llm_x[J][L][il][im], n * sizeof(double), err);
+  memcpy(b->ddJ_Llm_x[J][L][il][im], v, n * sizeof(double));
+  return _SUCCESS_;
+}
+
 /* Inverse of los_scenario_gather: write values into ptr2->transfer. */
 int los_scenario_scatter(los_scenario *s, const double *in, long n, char *err)
 {

@@ -75,6 +75,7 @@ struct los_stage {
   int ****xmin_J;     /* index_xmin_J[J][L][l][m] */
   int ****x_size_J;   /* x_size_J[J][L][l][m] */
   double *****J;      /* J_Llm_x[J][L][l][m][x - xmin] */
+  double *****ddJ;    /* ddJ_Llm_x, same indexing (cubic interpolation in x); NULL if linear */
 
   /* outputs (ptr2) */
   double ****transfer;       /* transfer[tt][k1][k2][k3] */

@@ -710,9 +710,11 @@ static hipError_t launch_sweep(const los_params *P, bool pers, bool pairs, int n
 // live L, in L order, of (a_J J_i + (1 - a_J) J_{i+1}) S; the integral adds
 // integrand x dtau; E adds the B->E mixing integral only for m != 0, B always
 // subtracts the E->B one; then x 1/2, / 4, / (2l+1), / 2
-// (source/transfer2.c:2259-2329, :1861-2098).
+// (source/transfer2.c:2259-2329, :1861-2098). CUBIC: J from the cubic form of
+// source/transfer2.c:2262-2268, a_J Jl + (1 - a_J)(Jr - a_J((a_J + 1) ddl +
+// (2 - a_J) ddr) step^2 / 6), with (dd[x], dd[x+1]) pairs zeroed as the J pairs.
 // ---------------------------------------------------------------------------
-template <int NK, int NL>
+template <int NK, int NL, bool CUBIC>
 __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, int q, int w)
 {
   if (q >= P.xcd_tile[xcd + 1] - P.xcd_tile[xcd]) return;
@@ -743,6 +745,8 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
   // sources) are skipped by wave-uniform tests.
   const bool mixB = P.jEB >= 0 && P.has_B;
   const __amdgpu_buffer_rsrc_t Jr = sweep_rsrc(P.Jp, P.Jp_bytes);
+  const __amdgpu_buffer_rsrc_t Dr = sweep_rsrc(CUBIC ? P.Jd : P.Jp, P.Jp_bytes);
+  const double step = P.xx_step;
   const unsigned lane = (unsigned)lp * 16u;
   const unsigned xstep = (unsigned)NLP * 16u;
   const unsigned rowb = (unsigned)P.X * xstep;
@@ -757,10 +761,21 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
   double accT[NK], accEd[NK], accEm[NK], accBd[NK], accBm[NK];
 #pragma unroll
   for (int j = 0; j < NK; ++j) accT[j] = accEd[j] = accEm[j] = accBd[j] = accBm[j] = 0;
-  // J rows at the current index_x (rows this wave never reads stay zero)
+  // J rows at the current index_x (rows this wave never reads stay zero), and
+  // their second derivatives for cubic J
   SweepPair<double> tv[NL], ev[NL], qv[NL];
+  SweepPair<double> td[CUBIC ? NL : 1], ed[CUBIC ? NL : 1], qd[CUBIC ? NL : 1];
 #pragma unroll
   for (int L = 0; L < NL; ++L) tv[L] = ev[L] = qv[L] = SweepPair<double>{0., 0.};
+#pragma unroll
+  for (int L = 0; L < (CUBIC ? NL : 1); ++L) td[L] = ed[L] = qd[L] = SweepPair<double>{0., 0.};
+  // J at a_J from a row's pair (and second-derivative pair), the reference's operations
+  auto jval = [&](double aJ, const SweepPair<double> &v, const SweepPair<double> &d) {
+    if (CUBIC)
+      return aJ * v.j0 + (1. - aJ) * (v.j1 - aJ * ((aJ + 1.) * d.j0 + (2. - aJ) * d.j1) * step *
+                                                 step / 6.0);
+    return aJ * v.j0 + (1 - aJ) * v.j1;
+  };
   // The per-step values that all lanes share come through the vector path and
   // are broadcast with readlane where they are used: lane j holds element j's
   // (index_x, a_J), read one step ahead, and lane i of src[j] holds value i of
@@ -794,10 +809,17 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
 #pragma unroll
         for (int L = 0; L < NL; ++L) {
           if (m > L) continue;
-          if (doT && L <= P.LmaxT) tv[L] = sweep_row<double>(Jr, lane, bT + L * rowb + ox, xstep);
+          if (doT && L <= P.LmaxT) {
+            tv[L] = sweep_row<double>(Jr, lane, bT + L * rowb + ox, xstep);
+            if (CUBIC) td[CUBIC ? L : 0] = sweep_row<double>(Dr, lane, bT + L * rowb + ox, xstep);
+          }
           if (doP && L <= P.LmaxP) {
             ev[L] = sweep_row<double>(Jr, lane, bE + L * rowb + ox, xstep);
             qv[L] = sweep_row<double>(Jr, lane, bQ + L * rowb + ox, xstep);
+            if (CUBIC) {
+              ed[CUBIC ? L : 0] = sweep_row<double>(Dr, lane, bE + L * rowb + ox, xstep);
+              qd[CUBIC ? L : 0] = sweep_row<double>(Dr, lane, bQ + L * rowb + ox, xstep);
+            }
           }
         }
       }
@@ -807,7 +829,7 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
 #pragma unroll
         for (int L = 0; L < NL; ++L)
           if (m <= L && L <= P.LmaxT)
-            integrand += (aJ * tv[L].j0 + (1 - aJ) * tv[L].j1) * sweep_lane(src[j], L);
+            integrand += jval(aJ, tv[L], td[CUBIC ? L : 0]) * sweep_lane(src[j], L);
         accT[j] += integrand * dt;
       }
       if (doP) {
@@ -817,12 +839,12 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
           if (m > L || L > P.LmaxP) continue;
           const double sE = sweep_lane(src[j], NL + L), sB = sweep_lane(src[j], 2 * NL + L);
           {
-            const double Jv = aJ * ev[L].j0 + (1 - aJ) * ev[L].j1;
+            const double Jv = jval(aJ, ev[L], ed[CUBIC ? L : 0]);
             if (P.has_E) iEd += Jv * sE;
             if (P.has_B) iBd += Jv * sB;
           }
           if (mixB) {
-            const double Jv = aJ * qv[L].j0 + (1 - aJ) * qv[L].j1;
+            const double Jv = jval(aJ, qv[L], qd[CUBIC ? L : 0]);
             if (m != 0) iEm += Jv * sB;
             iBm += Jv * sE;
           }
@@ -876,11 +898,11 @@ __device__ __forceinline__ void sweep_exact_item(const los_params &P, int xcd, i
   }
 }
 
-template <int NK, int NL, bool PERS>
+template <int NK, int NL, bool PERS, bool CUBIC = false>
 __global__ void __launch_bounds__(64)
 los_sweep_exact(los_params P)
 {
-#define LOS_CALL_EXACT(...) sweep_exact_item<NK, NL>(__VA_ARGS__)
+#define LOS_CALL_EXACT(...) sweep_exact_item<NK, NL, CUBIC>(__VA_ARGS__)
   LOS_SWEEP_ITEMS(LOS_CALL_EXACT)
 #undef LOS_CALL_EXACT
 }
@@ -890,18 +912,19 @@ static hipError_t launch_sweep_exact(const los_params *P, bool pers, int n_cu, h
 {
   if (P->n_pos == 0 || P->n_wave == 0) return hipSuccess;
   const int NL = (P->LmaxT > P->LmaxP ? P->LmaxT : P->LmaxP) + 1;
-  switch (NL * 2 + (pers ? 1 : 0)) {
-#define LOS_SWX1(n, b)                                                                \
-  case n * 2 + b: {                                                                   \
+  switch (NL * 4 + (pers ? 2 : 0) + (P->cubic_J ? 1 : 0)) {
+#define LOS_SWX1(n, b, c)                                                             \
+  case n * 4 + (b ? 2 : 0) + (c ? 1 : 0): {                                          \
     dim3 grid;                                                                        \
     const hipError_t e =                                                              \
-        sweep_grid(P, b, (const void *)los_sweep_exact<NK, n, b>, n_cu, &grid);       \
+        sweep_grid(P, b, (const void *)los_sweep_exact<NK, n, b, c>, n_cu, &grid);    \
     if (e != hipSuccess) return e;                                                    \
     if (grid.x == 0) return hipSuccess;                                               \
-    hipLaunchKernelGGL((los_sweep_exact<NK, n, b>), grid, dim3(64), 0, s, *P);        \
+    hipLaunchKernelGGL((los_sweep_exact<NK, n, b, c>), grid, dim3(64), 0, s, *P);     \
     break;                                                                            \
   }
-#define LOS_SWX(n) LOS_SWX1(n, false) LOS_SWX1(n, true)
+#define LOS_SWX(n) LOS_SWX1(n, false, false) LOS_SWX1(n, true, false) \
+                   LOS_SWX1(n, false, true) LOS_SWX1(n, true, true)
     LOS_SWX(1) LOS_SWX(2) LOS_SWX(3) LOS_SWX(4) LOS_SWX(5) LOS_SWX(6) LOS_SWX(7) LOS_SWX(8)
 #undef LOS_SWX
 #undef LOS_SWX1

@@ -749,6 +749,7 @@ static void stage_fill(los_stage *st, song_structs *q)
   st->xmin_J = pbs2->index_xmin_J;
   st->x_size_J = pbs2->x_size_J;
   st->J = pbs2->J_Llm_x;
+  st->ddJ = st->linear_J ? NULL : pbs2->ddJ_Llm_x;
   st->transfer = ptr2->transfer;
   st->has_transfers = ptr2->has_allocated_transfers;
   st->count_memorised = &ptr2->count_memorised_transfers;

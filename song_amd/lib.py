@@ -95,6 +95,7 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_scenario_override_k": (i, [vp, C.POINTER(d), i, ch]),
         "los_scenario_override_k3": (i, [vp, i, i, C.POINTER(d), i, ch]),
         "los_scenario_override_J": (i, [vp, i, i, i, i, i, C.POINTER(d), i, ch]),
+        "los_scenario_override_ddJ": (i, [vp, i, i, i, i, C.POINTER(d), i, ch]),
         "los_scenario_scatter": (i, [vp, C.POINTER(d), C.c_long, ch]),
         "los_scenario_k3_sizes": (i, [vp, C.POINTER(i), C.c_long, ch]),
         "los_scenario_store_sources": (i, [vp, ch]),
@@ -364,6 +365,14 @@ class Scenario:
         check(self.lib.los_scenario_override_J(self.ptr, jt, L, il, im, xmin,
                                                v.ctypes.data_as(C.POINTER(C.c_double)), v.size,
                                                err), err, "override_J")
+
+    def override_ddJ(self, jt: int, L: int, il: int, im: int, values) -> None:
+        """Second derivatives of one J row (cubic J interpolation), same size as the row."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        err = _err()
+        check(self.lib.los_scenario_override_ddJ(self.ptr, jt, L, il, im,
+                                                 v.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 v.size, err), err, "override_ddJ")
 
     def source(self, tp: int, i1: int, i2: int) -> np.ndarray:
         """Writable view of ppt2->sources[tp][i1][i2] as [tau][k3]."""

@@ -198,3 +198,33 @@ def test_small_batches_bit_exact(oracle, monkeypatch):
     got = s.gather()
     s.close()
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.gpu
+def test_cubic_J_bit_exact(oracle):
+    """Cubic interpolation of J in x (source/transfer2.c:2259-2271, second derivatives from
+    the J builder): transfer2_init in the plain form equals the oracle bit for bit, and
+    differs from the linear-J result (the cubic branch ran); the fast forms refuse it."""
+    kw = dict(CASES["TEB_m012"], bessels_interpolation=L.CUBIC)
+    cfg = L.Config(**kw)
+    print("case cubic_J", flush=True)
+    ref = run_oracle(L.Scenario(cfg), oracle)
+    lin = run_oracle(L.Scenario(L.Config(**CASES["TEB_m012"])), oracle)
+    s = L.Scenario(cfg)
+    s.transfer2_init()
+    got = s.gather()
+    s.close()
+    assert np.array_equal(got, ref)
+    assert not np.array_equal(got, lin)
+    s = L.Scenario(cfg)
+    s.prepare()
+    eng = L.Engine(0)
+    eng.load(s)
+    for form in (L.FORM_FP32C, L.FORM_FP64_FAST):
+        eng.set_form(form)
+        with pytest.raises(L.LosError, match="cubic"):
+            eng.autotune(1)
+        with pytest.raises(L.LosError, match="cubic"):
+            eng.run()
+    eng.close()
+    s.close()

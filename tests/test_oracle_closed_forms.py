@@ -348,3 +348,106 @@ def test_custom_sampling_closed_form(oracle, interp, planted):
     assert _custom_case(oracle, interp) < 1e-11
     if interp == L.LINEAR:
         assert _custom_case(planted(6), interp) > 1e-6
+
+
+# ---------------------------------------------------------------------------
+# cubic interpolation of J in x
+# ---------------------------------------------------------------------------
+def _cubic_j_case(oracle, seed=23, i0=36):
+    """J rows a + b x + g x^2 + h x^3 with their exact second derivatives 2 g + 6 h x, sources
+    that vanish except at one time step tau_i0 (constant in k3): the cubic form of
+    source/transfer2.c:2262-2268 is the cubic through the two nodes with the given second
+    derivatives, so it reproduces a cubic exactly, and each transfer is
+    dtau_i0 / 2 sum_L c J_L(k (tau0 - tau_i0)) / (8 (2l + 1)) with E = EE + [m != 0] EB and
+    B = EE - EB as in source/transfer2.c:1885-2061."""
+    cfg = L.Config(**dict(BASE, bessels_interpolation=L.CUBIC))
+    s = L.Scenario(cfg)
+    rng = np.random.default_rng(seed)
+    lvals, kvals, tau = s.grids()
+    m = cfg.m_list
+    xx_size = int(cfg.xx_max / cfg.xx_step + 1)
+    xx = np.arange(xx_size) * (cfg.xx_max / (xx_size - 1))
+    coef = {}
+    for jt_i, jt in enumerate(["TT", "EE", "EB"]):
+        for Lq in range(max(cfg.l_max_los_t, cfg.l_max_los_p) + 1):
+            for il, l in enumerate(lvals):
+                for im, mm in enumerate(m):
+                    if mm > min(Lq, l):
+                        continue
+                    # near the observer (x of a few units) the cubic term dominates and the
+                    # second derivatives differ between neighbouring nodes
+                    a, b = rng.normal() * 1e-2, rng.normal() * 1e-2
+                    g, h = rng.normal() * 1e-2, rng.normal() * 1e-2
+                    coef[(jt, Lq, il, im)] = (a, b, g, h)
+                    s.override_J(jt_i, Lq, il, im, 0, a + b * xx + g * xx * xx + h * xx ** 3)
+                    s.override_ddJ(jt_i, Lq, il, im, 2 * g + 6 * h * xx)
+    keys = [(f, Lq, mm) for f, Lm in (("T", cfg.l_max_los_t), ("E", cfg.l_max_los_p),
+                                       ("B", cfg.l_max_los_p))
+            for Lq in range(Lm + 1) for mm in m if mm <= Lq]
+    c = {}
+    for i1 in range(cfg.k_size):
+        for i2 in range(i1 + 1):
+            for t, key in enumerate(keys):
+                c[(i1, i2) + key] = rng.normal()
+                src = s.source(t, i1, i2)
+                src[:, :] = 0.0
+                src[i0, :] = c[(i1, i2) + key]
+    rc = oracle.los_ref_transfer2_init(*s.structs())
+    s._inited = True
+    assert rc == 0, s.error_message()
+    got = s.gather()
+    sizes = s.k3_sizes()
+    lists, q = {}, 0
+    for i1 in range(cfg.k_size):
+        for i2 in range(i1 + 1):
+            n = int(sizes[q, 0])
+            buf, last = np.zeros(n + 4), C.c_int()
+            st = s.s
+            assert oracle.los_ref_get_k3_list(st["ppr"], st["ppr2"], st["ppt2"], st["pbs2"],
+                                              st["ptr2"], i1, i2,
+                                              buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                              C.byref(last)) == 0
+            lists[(i1, i2)] = buf[:n]
+            q += 1
+    s.free_transfers(lambda a, b, c_: oracle.los_ref_transfer2_free(a, b, c_))
+    s.close()
+    dt = tau[i0 + 1] - tau[i0 - 1]
+    d0 = cfg.tau0 - tau[i0]
+
+    def J(jt, Lq, il, im, k):
+        a, b, g, h = coef[(jt, Lq, il, im)]
+        x = k * d0
+        return a + b * x + g * x * x + h * x ** 3
+
+    out = []
+    for f in ("T", "E", "B"):
+        Lm = cfg.l_max_los_t if f == "T" else cfg.l_max_los_p
+        for il, l in enumerate(lvals):
+            for im, mm in enumerate(m):
+                if mm > l:
+                    continue
+                for i1 in range(cfg.k_size):
+                    for i2 in range(i1 + 1):
+                        for k in lists[(i1, i2)]:
+                            Ls = range(mm, Lm + 1)
+                            cc = lambda F, Lq: c[(i1, i2, F, Lq, mm)]
+                            if f == "T":
+                                v = sum(cc("T", Lq) * J("TT", Lq, il, im, k) for Lq in Ls)
+                            elif f == "E":
+                                v = sum(cc("E", Lq) * J("EE", Lq, il, im, k) for Lq in Ls)
+                                if mm != 0:
+                                    v += sum(cc("B", Lq) * J("EB", Lq, il, im, k) for Lq in Ls)
+                            else:
+                                v = sum(cc("B", Lq) * J("EE", Lq, il, im, k) for Lq in Ls) - \
+                                    sum(cc("E", Lq) * J("EB", Lq, il, im, k) for Lq in Ls)
+                            out.append(0.5 * dt * v / (8.0 * (2 * l + 1)))
+    exp = np.array(out)
+    assert got.shape == exp.shape
+    return float(np.max(np.abs(got - exp)) / np.max(np.abs(exp)))
+
+
+def test_cubic_J_closed_form(oracle, planted):
+    good, bad = _cubic_j_case(oracle), _cubic_j_case(planted(7))
+    print(f"cubic J: oracle {good:.2e}, swapped weights {bad:.2e}")
+    assert good < 1e-11
+    assert bad > 1e-9  # 1e8 times the correct case
