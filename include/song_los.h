@@ -140,6 +140,13 @@ int los_engine_store(los_engine *eng, struct transfers2 *ptr2, char *errmsg);
 
 /* Element count (tt x fine k3 summed over loaded pairs). */
 long los_engine_n_elements(los_engine *eng);
+
+/* The last run's transfers in device memory, without a copy to the host: out =
+   [rows = tt2_size][cols = fine k3 of the loaded pairs, in load order], valid
+   until the next load or destroy. Hands the stage's output to a consumer on the
+   same device (e.g. los_cl2_sum_noalloc on the T or E rows). No counterpart in
+   SONG. Returns _FAILURE_ when nothing is loaded. */
+int los_engine_device_output(los_engine *eng, const double **out, long *rows, long *cols);
 /* Algorithmic operation count of one run of the loaded batch: per element and
    integral, sum over tau of 5 per live L (J interpolation 3, multiply-add 2)
    plus 2 when any L is live (trapezoid); `live` applies the reference's
@@ -149,9 +156,9 @@ int los_engine_flops(los_engine *eng, double *live, double *dense);
    timed): the kernel of known traffic that HBM counter readings are scaled by. */
 int los_engine_copy_probe(los_engine *eng, double bytes, float *ms, char *errmsg);
 /* Algorithmic bytes of one LOS kernel launch of the current form: the coarse
-   source columns once (twice with the cubic k3 spline's second derivatives),
-   the live part of the J table once (FP32 for the FP32-compensated form), the
-   outputs once. */
+   source values once (the spline's second derivatives are derived data and not
+   counted), the live part of the J table once (FP32 for the FP32-compensated
+   form), the outputs once. */
 double los_engine_alg_bytes(los_engine *eng);
 /* The HIP stream the engine launches on (hipStream_t as void*). */
 void *los_engine_stream(los_engine *eng);

@@ -1307,6 +1307,17 @@ extern "C" int los_engine_store_stage(los_engine *e, const los_stage *st, char *
 
 extern "C" long los_engine_n_elements(los_engine *e) { return (long)e->tt2_size * e->F; }
 
+// The last run's transfers where they are, in device memory: [tt][F], column f
+// = the loaded pairs' fine k3 in load order (each pair's grid in turn).
+extern "C" int los_engine_device_output(los_engine *e, const double **out, long *rows,
+                                        long *cols)
+{
+  if (out) *out = e->loaded ? e->dout : nullptr;
+  if (rows) *rows = e->loaded ? e->tt2_size : 0;
+  if (cols) *cols = e->loaded ? e->F : 0;
+  return e->loaded && e->dout ? _SUCCESS_ : _FAILURE_;
+}
+
 extern "C" hipError_t los_launch_copy_probe(const double *src, double *dst, long n, hipStream_t s);
 
 // One launch of a plain copy kernel of `bytes` on the engine's stream (device

@@ -138,6 +138,8 @@ def _declare(lib: C.CDLL) -> C.CDLL:
         "los_engine_flops": (i, [vp, C.POINTER(d), C.POINTER(d)]),
         "los_engine_copy_probe": (i, [vp, d, C.POINTER(C.c_float), ch]),
         "los_engine_alg_bytes": (d, [vp]),
+        "los_engine_device_output": (i, [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_long),
+                                         C.POINTER(C.c_long)]),
         "los_engine_stream": (vp, [vp]),
         "los_k3int_create": (i, [C.POINTER(CK3IntDesc), i, C.POINTER(vp), ch]),
         "los_k3int_run": (i, [vp, i, C.POINTER(C.c_float), ch]),
@@ -489,6 +491,13 @@ class Engine:
 
     def n_elements(self) -> int:
         return self.lib.los_engine_n_elements(self.ptr)
+
+    def device_output(self):
+        """(device address, rows, cols) of the last run's transfers, [tt][fine k3]."""
+        p, r, c = C.c_void_p(), C.c_long(), C.c_long()
+        if self.lib.los_engine_device_output(self.ptr, C.byref(p), C.byref(r), C.byref(c)):
+            raise LosError("los_engine_device_output: nothing loaded")
+        return p.value, r.value, c.value
 
     def alg_bytes(self) -> float:
         return self.lib.los_engine_alg_bytes(self.ptr)
