@@ -89,6 +89,9 @@ DESCR = {
 }
 
 FORMS = {"fp64": L.FORM_FP64, "fp64fast": L.FORM_FP64_FAST, "fp32c": L.FORM_FP32C}
+# error bars of the fast forms against the oracle: |T - T_oracle| / max |T_oracle| over each
+# (l,m) field; the row-normalised figure (per (tt, pair) row) is reported, not barred
+FIELD_BAR = {"fp64fast": 1e-12, "fp32c": 1e-5}
 POLICY = {"fp64fast": "StreamFP64", "fp32c": "StreamFP32C"}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a copy
 # kernel of known size), keyed by kernel name; the bench reports them for the kernel it ran
@@ -773,10 +776,15 @@ def main(argv=None):
                 dd = np.abs(rows_of(full[f], p) - rr)
                 re_.append(np.divide(dd, sc, out=np.zeros_like(dd), where=sc > 0))
             re_ = np.concatenate(re_, axis=1)
+            bar = FIELD_BAR[f]
             parity["forms"][f] = {"max_rel_err": float(fe.max()),
                                   "rms_rel_err": float(np.sqrt(np.mean(fe ** 2))),
                                   "max_rel_err_worst_tt": int(np.argmax(fe.max(axis=1))),
-                                  "max_row_rel_err": float(re_.max())}
+                                  "max_row_rel_err": float(re_.max()),
+                                  "field_bar": bar, "within_field_bar": bool(fe.max() < bar)}
+            if not fe.max() < bar:
+                print(f"# WARNING: {f} exceeds its bar against the oracle: {fe.max():.3e} >= "
+                      f"{bar:.0e}", file=sys.stderr, flush=True)
             per_lm_oracle[f] = {"max": fe.max(axis=1).tolist(),
                                 "rms": np.sqrt(np.mean(fe ** 2, axis=1)).tolist(),
                                 "row_max": re_.max(axis=1).tolist()}
